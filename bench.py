@@ -1,0 +1,200 @@
+#!/usr/bin/env python3
+"""Benchmark: seconds for CG to reach relative residual 1e-10 on the 27-point HPCCG matrix at
+256^3 (16.8M rows, 449M nnz) on one MI355X, plus the FP64 SpMV's HBM roofline.
+
+One step = one full solve (Prepare + iterations + final x update) from x0 = 0 on resident
+inputs: b = A x*, x* = 8 random low-frequency 3-D sines + U(-0.05, 0.05) noise (seed 14).
+The stopping test is the reference's (recursive residual, numerics.F90:109), with tolerance
+1e-10 * ||b||; the FP64 true residual ||b - A x|| / ||b|| of every result is reported.
+
+    python bench.py [--steps K] [--warmup W] [--grid N] [--no-cpu] [--gpus 1]
+
+Prints one JSON line (last line of stdout).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "hpccg-f90_amd"))
+
+METRIC = "s to rel-res 1e-10 @256^3 (learned/mixed vs FP64 CG); SpMV HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+SEED = 14
+REL_TOL = 1e-10
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def spmv_bytes(n, nnz, s_val=8, s_idx=4, s_x=8, s_y=8):
+    """Algorithmic bytes of one CSR SpMV: values + columns + row pointers + x once + y."""
+    return nnz * (s_val + s_idx) + (n + 1) * 4 + n * s_x + n * s_y
+
+
+def cpu_baseline(grid, b, gpu_niters, log_dir):
+    """The reference's own OpenMP build (oracle/_ref/omp, compiled from MPI/src) solving the same
+    system on this host; its "TIMER: Total" (the iteration loop) is the reported value."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle  # noqa: F401  (locates the reference build)
+    exe = oracle.ref_binary("omp", "ref_harness")
+    if exe is None:
+        return None
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    nx, ny, nz = grid
+    normb = math.sqrt(float(b @ b))
+    with tempfile.TemporaryDirectory() as td:
+        fb, fx = os.path.join(td, "b.bin"), os.path.join(td, "x.bin")
+        b.tofile(fb)
+        env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="spread", OMP_PLACES="cores")
+        t0 = time.perf_counter()
+        p = subprocess.run([exe, "solve", str(nx), str(ny), str(nz), "2000", repr(REL_TOL * normb), fb, fx],
+                           capture_output=True, text=True, env=env, timeout=600)
+        wall = time.perf_counter() - t0
+    if p.returncode != 0:
+        log("cpu baseline failed:", p.stderr[-2000:])
+        return None
+    total = None
+    niters = None
+    for ln in p.stdout.splitlines():
+        if "TIMER:" in ln and "Total" in ln:
+            total = float(ln.split("Total")[1].split()[0])
+        if ln.strip().startswith("H SOLVE"):
+            niters = int(ln.split()[2])
+    with open(os.path.join(log_dir, "cpu_baseline_stdout.txt"), "w") as f:
+        f.write(p.stdout)
+    return {"value": total, "unit": "s", "cores": cores, "kind": "reference",
+            "sample": f"full solve of the same {nx}x{ny}x{nz} system to 1e-10 by the reference cg "
+                      f"(MPI/src, amdflang -O3 -fopenmp -D_OMP), its TIMER: Total; {niters} iterations "
+                      f"(GPU fp64: {gpu_niters}); process wall incl. generate_matrix {wall:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--grid", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--spmv-reps", type=int, default=20)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the z-slab sharding design is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import hpccg_amd
+
+    n1 = args.grid
+    grid = (n1, n1, n1)
+    log_dir = os.path.join(ROOT, "gpurun_out")
+    os.makedirs(log_dir, exist_ok=True)
+
+    ctx = hpccg_amd.Context(0, hpccg_amd.DOT_TREE)
+    t0 = time.perf_counter()
+    ctx.generate(*grid)
+    info = ctx.matrix_info()
+    n, nnz = info["nrow"], info["nnz"]
+    xs_h = hpccg_amd.xstar_sines(*grid, SEED)
+    xs = hpccg_amd.DeviceArray.from_numpy(ctx, xs_h)
+    b = hpccg_amd.DeviceArray(ctx, n)
+    ctx.spmv_dev(xs, b)  # b = A x*, reference spmv order
+    b_h = b.numpy()
+    normb = math.sqrt(ctx.ddot_dev(b, b))
+    x = hpccg_amd.DeviceArray(ctx, n)
+    r = hpccg_amd.DeviceArray(ctx, n)
+    zeros = np.zeros(n)
+    log(f"setup {time.perf_counter() - t0:.1f} s: grid {grid}, n={n}, nnz={nnz}, ||b||={normb:.6e}")
+
+    def true_relres():
+        ctx.spmv_dev(x, r)
+        rr = r.numpy()
+        d = b_h - rr
+        return math.sqrt(float(d @ d)) / normb
+
+    def run_fp64():
+        x.upload(zeros)
+        res = ctx.cg_dev(b, x, max_iter=2000, tolerance=REL_TOL * normb, want_hist=False)
+        return res
+
+    solvers = {"fp64_cg": run_fp64}
+    headline = "fp64_cg"
+    results = {}
+    for name, fn in solvers.items():
+        for _ in range(args.warmup):
+            fn()
+        times, dev_ms = [], []
+        for _ in range(args.steps):
+            t = time.perf_counter()
+            res = fn()
+            times.append(time.perf_counter() - t)
+            dev_ms.append(res["ms"])
+        rel = true_relres()
+        xerr = float(np.max(np.abs(x.numpy() - xs_h)))
+        results[name] = {"s_wall": sum(times) / len(times), "s_device": sum(dev_ms) / len(dev_ms) / 1e3,
+                         "niters": res["niters"], "true_relres": rel, "xerr_max": xerr}
+        log(f"{name}: {results[name]}")
+
+    # roofline of the dominant kernel: the FP64 CSR SpMV as the CG loop launches it
+    y = hpccg_amd.DeviceArray(ctx, n)
+    spmv_ms = ctx.time_spmv(xs, y, reps=args.spmv_reps)
+    B = spmv_bytes(n, nnz)
+    achieved = B / (spmv_ms * 1e-3) / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", f"spmv_fp64_{n1}_traffic.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get("traffic_bytes")
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": "k_cg_spmv<128,true> (FP64 CSR SpMV + p.Ap epilogue)",
+                "algorithmic_bytes": B, "avg_ms": round(spmv_ms, 5)}
+    # calibration: a plain copy of the same byte count
+    cb = min(B // 2, 2 ** 31) // 16 * 16
+    ca = hpccg_amd.DeviceArray(ctx, cb // 8)
+    cbuf = hpccg_amd.DeviceArray(ctx, cb // 8)
+    copy_ms = ctx.time_copy(cbuf, ca, reps=20)
+    roofline["copy_GBs"] = round(2 * cb / (copy_ms * 1e-3) / 1e9, 1)
+    log(f"spmv {spmv_ms:.4f} ms -> {achieved:.0f} GB/s ({100 * achieved / HBM_PEAK_GBS:.1f}% of 8 TB/s); "
+        f"copy {roofline['copy_GBs']} GB/s")
+
+    head = results[headline]
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(grid, b_h, head["niters"], log_dir)
+        log(f"cpu baseline: {cpu}")
+
+    out = {
+        "metric": METRIC,
+        "value": round(head["s_wall"], 6),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["s_wall"] * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"cg-{n1}^3-sines-seed{SEED}-to-relres-1e-10", "grid": list(grid),
+                   "solver": headline, "rows": n, "nnz": nnz},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "solvers": results,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

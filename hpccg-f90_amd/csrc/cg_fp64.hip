@@ -1,0 +1,1196 @@
+// FP64 CG path of HPCCG-F90 on MI355X: CSR SpMV, dot products, vector updates, and the CG
+// iteration captured as a HIP graph.  Reference: MPI/src/numerics.F90:41-162,
+// MPI/src/SparseMatrix.F90:213-232, MPI/src/vector_operations.F90:40-169.
+//
+// Rounding contract.  The file is compiled with -ffp-contract=off: every product rounds
+// before it is added, exactly as the reference's serial loops do.
+//   * SpMV keeps the reference's per-row order (ascending k, from 0.0) in both dot modes, so
+//     y = A x is bitwise equal to the reference spmv.
+//   * waxpby / wwpby / wxbw are elementwise with the reference's operand order: bitwise.
+//   * Dots: HPCCG_DOT_SEQUENTIAL reproduces the reference's single left-to-right chain;
+//     HPCCG_DOT_TREE uses a fixed wave/workgroup/grid tree (fused multiply-adds inside).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <vector>
+
+#include "context.hpp"
+
+namespace hpccg {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+constexpr int kTileRows = 128;        // rows per SpMV tile == threads per SpMV workgroup
+constexpr int kTileCap = kTileRows * 27;  // products staged in LDS per tile (27-point rows)
+constexpr int kVecThreads = 256;
+constexpr int kMaxGrid = 16384;
+
+// ========================================================================================
+// SpMV tile: rows [tile*ROWS, tile*ROWS+ROWS).  Phase 1 streams the tile's contiguous
+// (value, column) range with unit-stride (coalesced) loads, gathers x, and stages the rounded
+// products in LDS; phase 2 sums each row's products in ascending k from 0.0 -- the reference's
+// order (SparseMatrix.F90:225-229).  Tiles whose nnz exceed the LDS capacity (rows longer than
+// 27 on average; never the case for generate_matrix's CSR) take a direct per-row loop with the
+// same order.  Returns x[r] * y[r] (the p.Ap term when x is p).
+// ========================================================================================
+template <int ROWS>
+__device__ __forceinline__ double spmv_tile(const int32_t *__restrict__ rowptr,
+                                            const int32_t *__restrict__ col,
+                                            const double *__restrict__ val,
+                                            const double *__restrict__ x, double *__restrict__ y,
+                                            int nrow, int tile, double *lds)
+{
+    const int t = threadIdx.x;
+    const int r0 = tile * ROWS;
+    const int r = r0 + t;
+    const int rlast = min(r0 + ROWS, nrow);
+    const int base = rowptr[r0] - 1;
+    const int end = rowptr[rlast] - 1;
+    int ms = 0, me = 0;
+    if (r < nrow) {
+        ms = rowptr[r] - 1;
+        me = rowptr[r + 1] - 1;
+    }
+    double sum = 0.0;
+    if (end - base <= ROWS * 27) {
+        for (int k = base + t; k < end; k += 4 * ROWS) {
+            double v[4];
+            int c[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int kk = k + u * ROWS;
+                if (kk < end) {
+                    v[u] = __builtin_nontemporal_load(val + kk);
+                    c[u] = __builtin_nontemporal_load(col + kk);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int kk = k + u * ROWS;
+                if (kk < end) lds[kk - base] = v[u] * x[c[u] - 1];
+            }
+        }
+        __syncthreads();
+        for (int j = ms; j < me; ++j) sum = sum + lds[j - base];
+        __syncthreads();
+    } else {
+        for (int j = ms; j < me; ++j) {
+            const double prod = val[j] * x[col[j] - 1];
+            sum = sum + prod;
+        }
+    }
+    if (r < nrow) {
+        y[r] = sum;
+        return x[r] * sum;
+    }
+    return 0.0;
+}
+
+// Contiguous tile range of one (XCD-renumbered) workgroup.
+__device__ __forceinline__ void block_range(int ntiles, int &t0, int &t1, int &lb)
+{
+    const int G = gridDim.x;
+    lb = xcd_logical_block(blockIdx.x, G);
+    t0 = (int)(((int64_t)lb * ntiles) / G);
+    t1 = (int)(((int64_t)(lb + 1) * ntiles) / G);
+}
+
+// y = A x  (standalone spmv; no reduction)
+template <int ROWS>
+__global__ __launch_bounds__(ROWS) void k_spmv(const int32_t *__restrict__ rowptr,
+                                               const int32_t *__restrict__ col,
+                                               const double *__restrict__ val,
+                                               const double *__restrict__ x, double *__restrict__ y,
+                                               int nrow, int ntiles)
+{
+    __shared__ double lds[ROWS * 27];
+    int t0, t1, lb;
+    block_range(ntiles, t0, t1, lb);
+    for (int tile = t0; tile < t1; ++tile) spmv_tile<ROWS>(rowptr, col, val, x, y, nrow, tile, lds);
+}
+
+// ---- CG control logic shared by the tree and sequential variants ---------------------------
+__device__ __forceinline__ bool cg_stopped(const CgState *st)
+{
+    return st->k >= st->stop_at;
+}
+
+// after p.Ap of iteration k: alpha = rtrans / (p.Ap)          numerics.F90:135-138
+__device__ __forceinline__ void cg_after_pap(CgState *st, const double *rr_hist, int k, double pap)
+{
+    if (!(pap != 0.0) || !isfinite(pap)) st->breakdown = 1;
+    st->alpha = rr_hist[k - 1] / pap;
+}
+
+// after r.r of iteration k (k = 0: Prepare).  Records ||r_k||^2, applies the reference's
+// lagging test (the top of iteration k+2 compares ||r_k|| with the tolerance; Prepare's
+// ||r_0|| is tested at the top of iteration 1), forms beta for iteration k+1 and advances k.
+__device__ __forceinline__ void cg_after_rr(CgState *st, double *rr_hist, int k, double rr)
+{
+    rr_hist[k] = rr;
+    const double normr = sqrt(rr);
+    if (k == 0) {
+        if (normr < st->tol) st->stop_at = 1;
+        st->k = 1;
+        return;
+    }
+    if (normr < st->tol) st->stop_at = min(st->stop_at, k + 2);
+    st->beta = rr / rr_hist[k - 1];  // numerics.F90:117-119
+    st->k = k + 1;
+}
+
+// ---- CG kernels ----------------------------------------------------------------------------
+// Ap = A p and (TREE) p.Ap -> alpha                       numerics.F90:133-138
+template <int ROWS, bool TREE>
+__global__ __launch_bounds__(ROWS) void k_cg_spmv(const int32_t *__restrict__ rowptr,
+                                                  const int32_t *__restrict__ col,
+                                                  const double *__restrict__ val,
+                                                  const double *__restrict__ p,
+                                                  double *__restrict__ Ap, int nrow, int ntiles,
+                                                  CgState *st, double *rr_hist, double *partials)
+{
+    __shared__ double lds[ROWS * 27];
+    __shared__ double red[ROWS / kWave];
+    const int k = st->k;
+    if (k >= st->stop_at) return;
+    int t0, t1, lb;
+    block_range(ntiles, t0, t1, lb);
+    double dot = 0.0;
+    for (int tile = t0; tile < t1; ++tile)
+        dot += spmv_tile<ROWS>(rowptr, col, val, p, Ap, nrow, tile, lds);
+    if (TREE) {
+        dot = block_sum<ROWS>(dot, red);
+        double tot;
+        if (grid_sum<ROWS>(dot, partials, &st->counter[0], gridDim.x, lb, red, &tot) && threadIdx.x == 0)
+            cg_after_pap(st, rr_hist, k, tot);
+    }
+}
+
+// Prepare: Ap = A x (p = x, numerics.F90:92-96)
+template <int ROWS>
+__global__ __launch_bounds__(ROWS) void k_cg_prep_spmv(const int32_t *__restrict__ rowptr,
+                                                       const int32_t *__restrict__ col,
+                                                       const double *__restrict__ val,
+                                                       const double *__restrict__ x,
+                                                       double *__restrict__ Ap, int nrow, int ntiles)
+{
+    __shared__ double lds[ROWS * 27];
+    int t0, t1, lb;
+    block_range(ntiles, t0, t1, lb);
+    for (int tile = t0; tile < t1; ++tile) spmv_tile<ROWS>(rowptr, col, val, x, Ap, nrow, tile, lds);
+}
+
+// Element range [i0, i1) of one workgroup in a contiguous split of n elements.
+__device__ __forceinline__ void elem_range(int64_t n, int64_t &i0, int64_t &i1, int &lb)
+{
+    const int G = gridDim.x;
+    lb = xcd_logical_block(blockIdx.x, G);
+    i0 = (n * lb) / G;
+    i1 = (n * (lb + 1)) / G;
+}
+
+// Prepare: r = 1*b + (-1)*Ap and (TREE) r.r              numerics.F90:97-99
+template <bool TREE>
+__global__ __launch_bounds__(kVecThreads) void k_cg_prep_r(const double *__restrict__ b,
+                                                           const double *__restrict__ Ap,
+                                                           double *__restrict__ r, int n,
+                                                           CgState *st, double *rr_hist,
+                                                           double *partials)
+{
+    __shared__ double red[kVecThreads / kWave];
+    int64_t i0, i1;
+    int lb;
+    elem_range(n, i0, i1, lb);
+    double dot = 0.0;
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += kVecThreads) {
+        const double t1 = 1.0 * b[i];
+        const double t2 = -1.0 * Ap[i];
+        const double ri = t1 + t2;
+        r[i] = ri;
+        dot = fma(ri, ri, dot);
+    }
+    if (TREE) {
+        dot = block_sum<kVecThreads>(dot, red);
+        double tot;
+        if (grid_sum<kVecThreads>(dot, partials, &st->counter[1], gridDim.x, lb, red, &tot) && threadIdx.x == 0)
+            cg_after_rr(st, rr_hist, 0, tot);
+    }
+}
+
+// r = r + (-alpha)*Ap and (TREE) r.r -> beta, test, k++    numerics.F90:139, :117-119
+template <bool TREE>
+__global__ __launch_bounds__(kVecThreads) void k_cg_update_r(double *__restrict__ r,
+                                                             const double *__restrict__ Ap, int n,
+                                                             CgState *st, double *rr_hist,
+                                                             double *partials)
+{
+    __shared__ double red[kVecThreads / kWave];
+    const int k = st->k;
+    if (k >= st->stop_at) return;
+    const double nalpha = -st->alpha;
+    int64_t i0, i1;
+    int lb;
+    elem_range(n, i0, i1, lb);
+    double dot = 0.0;
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += kVecThreads) {
+        const double t = nalpha * Ap[i];
+        const double ri = r[i] + t;
+        r[i] = ri;
+        dot = fma(ri, ri, dot);
+    }
+    if (TREE) {
+        dot = block_sum<kVecThreads>(dot, red);
+        double tot;
+        if (grid_sum<kVecThreads>(dot, partials, &st->counter[2], gridDim.x, lb, red, &tot) && threadIdx.x == 0)
+            cg_after_rr(st, rr_hist, k, tot);
+    }
+}
+
+// Top of iteration k: k == 1: p = r (numerics.F90:114-115); else x = x + alpha_{k-1} p_{k-1}
+// (the x update of iteration k-1, numerics.F90:140, deferred to here so that p is read once)
+// and p = r + beta p (numerics.F90:120).
+__global__ __launch_bounds__(kVecThreads) void k_cg_update_p(const double *__restrict__ r,
+                                                             double *__restrict__ p,
+                                                             double *__restrict__ x, int n,
+                                                             const CgState *st)
+{
+    const int k = st->k;
+    if (k >= st->stop_at) return;
+    int64_t i0, i1;
+    int lb;
+    elem_range(n, i0, i1, lb);
+    if (k == 1) {
+        for (int64_t i = i0 + threadIdx.x; i < i1; i += kVecThreads) p[i] = r[i];
+        return;
+    }
+    const double alpha = st->alpha, beta = st->beta;
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += kVecThreads) {
+        const double pi = p[i];
+        const double tx = alpha * pi;
+        x[i] = x[i] + tx;
+        const double tp = beta * pi;
+        p[i] = r[i] + tp;
+    }
+}
+
+// After the loop: the x update of the last iteration that ran.
+__global__ __launch_bounds__(kVecThreads) void k_cg_finish_x(const double *__restrict__ p,
+                                                             double *__restrict__ x, int n,
+                                                             const CgState *st)
+{
+    if (st->k - 1 < 1) return;
+    const double alpha = st->alpha;
+    int64_t i0, i1;
+    int lb;
+    elem_range(n, i0, i1, lb);
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += kVecThreads) {
+        const double tx = alpha * p[i];
+        x[i] = x[i] + tx;
+    }
+}
+
+// ---- sequential dots: the reference's single left-to-right chain --------------------------
+// One wavefront.  Lanes load 4x64 consecutive elements and form the rounded products; the
+// running sum then walks them in index order (v_readlane broadcasts), one rounded add each.
+__device__ __forceinline__ double readlane_f64(double v, int lane)
+{
+    const uint64_t u = __double_as_longlong(v);
+    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)u, lane);
+    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(u >> 32), lane);
+    return __longlong_as_double(((uint64_t)hi << 32) | lo);
+}
+
+template <bool SELF>
+__device__ double serial_dot(const double *__restrict__ x, const double *__restrict__ y, int n)
+{
+    const int lane = threadIdx.x;
+    double s = 0.0;
+    int base = 0;
+    for (; base + 4 * kWave <= n; base += 4 * kWave) {
+        double pr[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const double a = x[base + u * kWave + lane];
+            const double b = SELF ? a : y[base + u * kWave + lane];
+            pr[u] = a * b;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int j = 0; j < kWave; ++j) s = s + readlane_f64(pr[u], j);
+    }
+    for (; base < n; base += kWave) {
+        const int i = base + lane;
+        double pr = 0.0;
+        if (i < n) {
+            const double a = x[i];
+            const double b = SELF ? a : y[i];
+            pr = a * b;
+        }
+        const int cnt = min(kWave, n - base);
+        for (int j = 0; j < cnt; ++j) s = s + readlane_f64(pr, j);
+    }
+    return s;
+}
+
+__global__ __launch_bounds__(kWave) void k_dot_serial(const double *x, const double *y, int n,
+                                                      double *out)
+{
+    const double s = serial_dot<false>(x, y, n);
+    if (threadIdx.x == 0) *out = s;
+}
+
+__global__ __launch_bounds__(kWave) void k_cg_serial_pap(const double *p, const double *Ap, int n,
+                                                         CgState *st, const double *rr_hist)
+{
+    const int k = st->k;
+    if (k >= st->stop_at) return;
+    const double s = serial_dot<false>(p, Ap, n);
+    if (threadIdx.x == 0) cg_after_pap(st, rr_hist, k, s);
+}
+
+__global__ __launch_bounds__(kWave) void k_cg_serial_rr(const double *r, int n, CgState *st,
+                                                        double *rr_hist, int prep)
+{
+    const int k = prep ? 0 : st->k;
+    if (!prep && k >= st->stop_at) return;
+    const double s = serial_dot<true>(r, r, n);
+    if (threadIdx.x == 0) cg_after_rr(st, rr_hist, k, s);
+}
+
+// ---- standalone tree dot and vector ops -----------------------------------------------------
+__global__ __launch_bounds__(kVecThreads) void k_dot_tree(const double *__restrict__ x,
+                                                          const double *__restrict__ y, int n,
+                                                          double *partials, unsigned *counter,
+                                                          double *out)
+{
+    __shared__ double red[kVecThreads / kWave];
+    int64_t i0, i1;
+    int lb;
+    elem_range(n, i0, i1, lb);
+    double dot = 0.0;
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += kVecThreads) dot = fma(x[i], y[i], dot);
+    dot = block_sum<kVecThreads>(dot, red);
+    double tot;
+    if (grid_sum<kVecThreads>(dot, partials, counter, gridDim.x, lb, red, &tot) && threadIdx.x == 0)
+        *out = tot;
+}
+
+// w = alpha*x + beta*y      vector_operations.F90:116-119
+__global__ void k_waxpby(int n, double alpha, const double *__restrict__ x, double beta,
+                         const double *__restrict__ y, double *__restrict__ w)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double t1 = alpha * x[i];
+        const double t2 = beta * y[i];
+        w[i] = t1 + t2;
+    }
+}
+// w = w + beta*y            vector_operations.F90:132-135
+__global__ void k_wwpby(int n, double beta, const double *__restrict__ y, double *__restrict__ w)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double t = beta * y[i];
+        w[i] = w[i] + t;
+    }
+}
+// w = x + beta*w            vector_operations.F90:148-151
+__global__ void k_wxbw(int n, const double *__restrict__ x, double beta, double *__restrict__ w)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double t = beta * w[i];
+        w[i] = x[i] + t;
+    }
+}
+
+// plain copy: the HBM calibration kernel (16-byte accesses)
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_copy16(const u32x4 *__restrict__ src, u32x4 *__restrict__ dst, int64_t n16)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n16; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = __builtin_nontemporal_load(src + i);
+}
+
+// ---- device generate_matrix ---------------------------------------------------------------
+// nnz of all rows before coordinate m along one axis of extent n (1-D neighbour counts are 3,
+// except 2 at each end, and 1 when n == 1).
+__device__ __forceinline__ int64_t axis_prefix(int64_t m, int64_t n)
+{
+    return 3 * m - (m >= 1 ? 1 : 0) - (m >= n ? 1 : 0);
+}
+__device__ __forceinline__ int64_t axis_count(int64_t i, int64_t n)
+{
+    return axis_prefix(i + 1, n) - axis_prefix(i, n);
+}
+
+// One thread per row; same row order, column order and values as SparseMatrix.F90:138-175.
+__global__ void k_generate(int nx, int ny, int nz, int32_t *__restrict__ rowindex,
+                           int32_t *__restrict__ colindex, double *__restrict__ values,
+                           double *__restrict__ x, double *__restrict__ b, double *__restrict__ xexact)
+{
+    const int64_t N = (int64_t)nx * ny * nz;
+    const int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (row >= N) return;
+    const int ix = (int)(row % nx), iy = (int)((row / nx) % ny), iz = (int)(row / ((int64_t)nx * ny));
+    const int64_t cx = 3 * (int64_t)nx - 2, cy = 3 * (int64_t)ny - 2;
+    const int64_t cursor = axis_prefix(iz, nz) * cy * cx + axis_count(iz, nz) * axis_prefix(iy, ny) * cx +
+                           axis_count(iz, nz) * axis_count(iy, ny) * axis_prefix(ix, nx);
+    rowindex[row] = (int32_t)(cursor + 1);
+    int64_t c = cursor;
+    for (int sz = -1; sz <= 1; ++sz) {
+        if (iz + sz < 0 || iz + sz >= nz) continue;
+        for (int sy = -1; sy <= 1; ++sy) {
+            if (iy + sy < 0 || iy + sy >= ny) continue;
+            for (int sx = -1; sx <= 1; ++sx) {
+                if (ix + sx < 0 || ix + sx >= nx) continue;
+                const int64_t curcol = row + (int64_t)sz * nx * ny + (int64_t)sy * nx + sx;
+                values[c] = (curcol == row) ? 27.0 : -1.0;
+                colindex[c] = (int32_t)(curcol + 1);
+                ++c;
+            }
+        }
+    }
+    const int nnzrow = (int)(c - cursor);
+    x[row] = 0.0;
+    b[row] = 27.0 - (double)(nnzrow - 1);
+    xexact[row] = 1.0;
+    if (row == N - 1) rowindex[N] = (int32_t)(c + 1);
+}
+
+// ============================================================================================
+// host side
+// ============================================================================================
+static int spmv_grid(const hpccg_ctx *c, int ntiles)
+{
+    // 4 workgroups of 128 threads per CU keep LDS (27.6 KB each) and waves balanced
+    int G = std::min(ntiles, c->info.num_cu * 8);
+    G = std::max(kNumXcd, (G / kNumXcd) * kNumXcd);
+    return G;
+}
+static int vec_grid(const hpccg_ctx *c, int64_t n)
+{
+    int64_t G = std::min<int64_t>((n + kVecThreads - 1) / kVecThreads, (int64_t)c->info.num_cu * 4);
+    G = std::max<int64_t>(kNumXcd, (G / kNumXcd) * kNumXcd);
+    return (int)G;
+}
+
+static int ensure_workspace(hpccg_ctx *c, int64_t n)
+{
+    if (c->ws_n >= n && c->r) return 0;
+    hipFree(c->r);
+    hipFree(c->p);
+    hipFree(c->Ap);
+    c->r = c->p = c->Ap = nullptr;
+    c->ws_n = 0;
+    for (auto &g : c->graphs) hipGraphExecDestroy(g.second);
+    c->graphs.clear();
+    if (hipMalloc(&c->r, n * sizeof(double)) != hipSuccess || hipMalloc(&c->p, n * sizeof(double)) != hipSuccess ||
+        hipMalloc(&c->Ap, n * sizeof(double)) != hipSuccess) {
+        set_error("ensure_workspace: hipMalloc of %lld doubles failed", (long long)(3 * n));
+        return HPCCG_ERR_NOMEM;
+    }
+    c->ws_n = n;
+    return 0;
+}
+
+static int ensure_stage(hpccg_ctx *c, int64_t n)
+{
+    if (c->stage_n >= n && c->s0) return 0;
+    hipFree(c->s0);
+    hipFree(c->s1);
+    hipFree(c->s2);
+    c->s0 = c->s1 = c->s2 = nullptr;
+    c->stage_n = 0;
+    if (hipMalloc(&c->s0, n * sizeof(double)) != hipSuccess || hipMalloc(&c->s1, n * sizeof(double)) != hipSuccess ||
+        hipMalloc(&c->s2, n * sizeof(double)) != hipSuccess) {
+        set_error("ensure_stage: hipMalloc failed");
+        return HPCCG_ERR_NOMEM;
+    }
+    c->stage_n = n;
+    return 0;
+}
+
+static int ensure_hist(hpccg_ctx *c, int need)
+{
+    if (c->hist_cap >= need) return 0;
+    hipFree(c->rr_hist);
+    c->rr_hist = nullptr;
+    c->hist_cap = 0;
+    for (auto &g : c->graphs) hipGraphExecDestroy(g.second);
+    c->graphs.clear();
+    if (hipMalloc(&c->rr_hist, (size_t)need * sizeof(double)) != hipSuccess) {
+        set_error("ensure_hist: hipMalloc failed");
+        return HPCCG_ERR_NOMEM;
+    }
+    c->hist_cap = need;
+    return 0;
+}
+
+static int matrix_required(const hpccg_ctx *c)
+{
+    if (!c->A.rowindex) {
+        set_error("no matrix: call hpccg_matrix_upload or hpccg_generate_matrix first");
+        return HPCCG_ERR_STATE;
+    }
+    return 0;
+}
+
+static int ntiles_of(const hpccg_ctx *c) { return (c->A.nrow + kTileRows - 1) / kTileRows; }
+
+int launch_spmv(hpccg_ctx *c, const double *x, double *y)
+{
+    const int nt = ntiles_of(c);
+    if (nt == 0) return 0;
+    hipLaunchKernelGGL(k_spmv<kTileRows>, dim3(spmv_grid(c, nt)), dim3(kTileRows), 0, c->stream, c->A.rowindex,
+                       c->A.colindex, c->A.values, x, y, c->A.nrow, nt);
+    HPCCG_HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+static int launch_dot(hpccg_ctx *c, int n, const double *x, const double *y, double *out_dev)
+{
+    if (c->dot_mode == HPCCG_DOT_SEQUENTIAL) {
+        hipLaunchKernelGGL(k_dot_serial, dim3(1), dim3(kWave), 0, c->stream, x, y, n, out_dev);
+    } else {
+        hipLaunchKernelGGL(k_dot_tree, dim3(vec_grid(c, n)), dim3(kVecThreads), 0, c->stream, x, y, n, c->partials,
+                           &c->state->counter[3], out_dev);
+    }
+    HPCCG_HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// Enqueue `count` CG iterations (each skips itself on the device once the loop has ended).
+static int enqueue_iterations(hpccg_ctx *c, double *x, int count)
+{
+    const int n = c->A.nrow, nt = ntiles_of(c);
+    const int Gs = spmv_grid(c, nt), Gv = vec_grid(c, n);
+    const bool tree = c->dot_mode == HPCCG_DOT_TREE;
+    for (int it = 0; it < count; ++it) {
+        hipLaunchKernelGGL(k_cg_update_p, dim3(Gv), dim3(kVecThreads), 0, c->stream, c->r, c->p, x, n, c->state);
+        if (tree) {
+            hipLaunchKernelGGL((k_cg_spmv<kTileRows, true>), dim3(Gs), dim3(kTileRows), 0, c->stream, c->A.rowindex,
+                               c->A.colindex, c->A.values, c->p, c->Ap, n, nt, c->state, c->rr_hist, c->partials);
+            hipLaunchKernelGGL(k_cg_update_r<true>, dim3(Gv), dim3(kVecThreads), 0, c->stream, c->r, c->Ap, n,
+                               c->state, c->rr_hist, c->partials);
+        } else {
+            hipLaunchKernelGGL((k_cg_spmv<kTileRows, false>), dim3(Gs), dim3(kTileRows), 0, c->stream, c->A.rowindex,
+                               c->A.colindex, c->A.values, c->p, c->Ap, n, nt, c->state, c->rr_hist, c->partials);
+            hipLaunchKernelGGL(k_cg_serial_pap, dim3(1), dim3(kWave), 0, c->stream, c->p, c->Ap, n, c->state,
+                               c->rr_hist);
+            hipLaunchKernelGGL(k_cg_update_r<false>, dim3(Gv), dim3(kVecThreads), 0, c->stream, c->r, c->Ap, n,
+                               c->state, c->rr_hist, c->partials);
+            hipLaunchKernelGGL(k_cg_serial_rr, dim3(1), dim3(kWave), 0, c->stream, c->r, n, c->state, c->rr_hist, 0);
+        }
+    }
+    HPCCG_HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+static int get_chunk_graph(hpccg_ctx *c, const double *b, double *x, int chunk, hipGraphExec_t *out)
+{
+    GraphKey key{b, x, c->dot_mode, chunk};
+    auto it = c->graphs.find(key);
+    if (it != c->graphs.end()) {
+        *out = it->second;
+        return 0;
+    }
+    hipGraph_t g;
+    HPCCG_HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int st = enqueue_iterations(c, x, chunk);
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (st) return st;
+    if (e != hipSuccess) {
+        set_error("hipStreamEndCapture: %s", hipGetErrorString(e));
+        return HPCCG_ERR_HIP;
+    }
+    hipGraphExec_t ge;
+    e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        set_error("hipGraphInstantiate: %s", hipGetErrorString(e));
+        return HPCCG_ERR_HIP;
+    }
+    c->graphs[key] = ge;
+    *out = ge;
+    return 0;
+}
+
+// The whole solve on device-resident b and x.
+static int cg_device(hpccg_ctx *c, const double *b, double *x, int max_iter, double tol, int *niters,
+                     double *normr, double *hist, double *elapsed_ms)
+{
+    HPCCG_TRY(matrix_required(c));
+    HPCCG_CHECK_ARG(max_iter >= 0, "max_iter must be >= 0");
+    const int n = c->A.nrow;
+    HPCCG_TRY(ensure_workspace(c, n));
+    HPCCG_TRY(ensure_hist(c, max_iter + 2));
+    const int nt = ntiles_of(c);
+    const bool tree = c->dot_mode == HPCCG_DOT_TREE;
+
+    CgState init{};
+    init.tol = tol;
+    init.k = 0;
+    init.stop_at = max_iter + 1;
+    init.max_iter = max_iter;
+    *c->state_h = init;
+    HPCCG_HIP_TRY(hipMemcpyAsync(c->state, c->state_h, sizeof(CgState), hipMemcpyHostToDevice, c->stream));
+
+    HPCCG_HIP_TRY(hipEventRecord(c->ev0, c->stream));
+    // Prepare (numerics.F90:92-99): Ap = A x; r = b - Ap; rtrans = r.r
+    hipLaunchKernelGGL(k_cg_prep_spmv<kTileRows>, dim3(spmv_grid(c, nt)), dim3(kTileRows), 0, c->stream,
+                       c->A.rowindex, c->A.colindex, c->A.values, x, c->Ap, n, nt);
+    if (tree) {
+        hipLaunchKernelGGL(k_cg_prep_r<true>, dim3(vec_grid(c, n)), dim3(kVecThreads), 0, c->stream, b, c->Ap, c->r,
+                           n, c->state, c->rr_hist, c->partials);
+    } else {
+        hipLaunchKernelGGL(k_cg_prep_r<false>, dim3(vec_grid(c, n)), dim3(kVecThreads), 0, c->stream, b, c->Ap,
+                           c->r, n, c->state, c->rr_hist, c->partials);
+        hipLaunchKernelGGL(k_cg_serial_rr, dim3(1), dim3(kWave), 0, c->stream, c->r, n, c->state, c->rr_hist, 1);
+    }
+    HPCCG_HIP_TRY(hipGetLastError());
+
+    // Iteration loop ("Total" timer, numerics.F90:106-150): graphs of `chunk` iterations.
+    const int chunk = std::max(1, std::min(max_iter, tree ? 16 : 4));
+    hipGraphExec_t ge = nullptr;
+    if (max_iter > 0) HPCCG_TRY(get_chunk_graph(c, b, x, chunk, &ge));
+    int launched = 0;
+    bool done = max_iter == 0;
+    const bool can_stop_early = tol > 0.0 || std::isnan(tol);
+    while (!done) {
+        HPCCG_HIP_TRY(hipGraphLaunch(ge, c->stream));
+        launched += chunk;
+        if (launched >= max_iter) break;
+        if (can_stop_early) {
+            // read the state of the previous chunk while this one runs
+            HPCCG_HIP_TRY(hipMemcpyAsync(c->state_h, c->state, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
+            HPCCG_HIP_TRY(hipStreamSynchronize(c->stream));
+            done = c->state_h->k >= c->state_h->stop_at;
+        }
+    }
+    hipLaunchKernelGGL(k_cg_finish_x, dim3(vec_grid(c, n)), dim3(kVecThreads), 0, c->stream, c->p, x, n, c->state);
+    HPCCG_HIP_TRY(hipGetLastError());
+    HPCCG_HIP_TRY(hipEventRecord(c->ev1, c->stream));
+    HPCCG_HIP_TRY(hipMemcpyAsync(c->state_h, c->state, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
+    std::vector<double> rr(max_iter + 1);
+    HPCCG_HIP_TRY(hipMemcpyAsync(rr.data(), c->rr_hist, (max_iter + 1) * sizeof(double), hipMemcpyDeviceToHost,
+                                 c->stream));
+    HPCCG_HIP_TRY(hipStreamSynchronize(c->stream));
+    if (elapsed_ms) {
+        float ms = 0.f;
+        HPCCG_HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        *elapsed_ms = ms;
+    }
+    const CgState &s = *c->state_h;
+    const int nit = s.k - 1;
+    if (niters) *niters = nit;
+    // normr = ||r_{niters-1}|| (numerics.F90:122); with max_iter == 0 it stays ||r_0||
+    if (normr) *normr = std::sqrt(rr[std::max(0, nit - 1)]);
+    if (hist) {
+        for (int j = 0; j <= max_iter; ++j) hist[j] = (j <= nit) ? std::sqrt(rr[j]) : -1.0;
+    }
+    return s.breakdown ? HPCCG_WARN_BREAKDOWN : 0;
+}
+
+}  // namespace hpccg
+
+using namespace hpccg;
+
+// ============================================================================================
+// C ABI
+// ============================================================================================
+extern "C" {
+
+const char *hpccg_last_error(void) { return g_err; }
+
+int hpccg_create(hpccg_ctx **out, int device)
+{
+    HPCCG_CHECK_ARG(out, "null handle pointer");
+    *out = nullptr;
+    int count = 0;
+    HPCCG_HIP_TRY(hipGetDeviceCount(&count));
+    HPCCG_CHECK_ARG(device >= 0 && device < count, "device index out of range");
+    HPCCG_HIP_TRY(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HPCCG_HIP_TRY(hipGetDeviceProperties(&prop, device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        set_error("hpccg_create: device %d is %s; this library is built for gfx950 (MI355X) only", device,
+                  prop.gcnArchName);
+        return HPCCG_ERR_ARG;
+    }
+    auto *c = new hpccg_ctx();
+    c->device = device;
+    c->info.num_cu = prop.multiProcessorCount;
+    c->info.gfx = 950;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc(&c->state, sizeof(CgState)) != hipSuccess ||
+        hipHostMalloc(&c->state_h, sizeof(CgState), hipHostMallocDefault) != hipSuccess ||
+        hipMalloc(&c->partials, kMaxGrid * sizeof(double)) != hipSuccess ||
+        hipMalloc(&c->scalar, 8 * sizeof(double)) != hipSuccess ||
+        hipHostMalloc(&c->scalar_h, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        set_error("hpccg_create: HIP resource allocation failed");
+        hpccg_destroy(c);
+        return HPCCG_ERR_HIP;
+    }
+    // counters start at zero (a fill kernel would do as well; this runs once, never captured)
+    CgState zero{};
+    memcpy(c->state_h, &zero, sizeof(zero));
+    if (hipMemcpy(c->state, c->state_h, sizeof(CgState), hipMemcpyHostToDevice) != hipSuccess) {
+        hpccg_destroy(c);
+        set_error("hpccg_create: state init failed");
+        return HPCCG_ERR_HIP;
+    }
+    *out = c;
+    return 0;
+}
+
+void hpccg_free_ext(hpccg_ctx *c);  // mixed / batched / learned paths release their buffers
+
+int hpccg_destroy(hpccg_ctx *c)
+{
+    if (!c) return 0;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    hpccg_free_ext(c);
+    for (auto &g : c->graphs) hipGraphExecDestroy(g.second);
+    hipFree(c->A.rowindex);
+    hipFree(c->A.colindex);
+    hipFree(c->A.values);
+    hipFree(c->A.values16);
+    hipFree(c->r);
+    hipFree(c->p);
+    hipFree(c->Ap);
+    hipFree(c->state);
+    hipHostFree(c->state_h);
+    hipFree(c->rr_hist);
+    hipFree(c->partials);
+    hipFree(c->scalar);
+    hipHostFree(c->scalar_h);
+    hipFree(c->s0);
+    hipFree(c->s1);
+    hipFree(c->s2);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return 0;
+}
+
+int hpccg_set_dot_mode(hpccg_ctx *c, int mode)
+{
+    HPCCG_CHECK_ARG(c, "null context");
+    HPCCG_CHECK_ARG(mode == HPCCG_DOT_SEQUENTIAL || mode == HPCCG_DOT_TREE, "unknown dot mode");
+    c->dot_mode = mode;
+    return 0;
+}
+
+int hpccg_get_dot_mode(hpccg_ctx *c, int *mode)
+{
+    HPCCG_CHECK_ARG(c && mode, "null argument");
+    *mode = c->dot_mode;
+    return 0;
+}
+
+int hpccg_device_info(hpccg_ctx *c, int *num_cu, int *num_xcd, int *gfx)
+{
+    HPCCG_CHECK_ARG(c, "null context");
+    if (num_cu) *num_cu = c->info.num_cu;
+    if (num_xcd) *num_xcd = kNumXcd;
+    if (gfx) *gfx = c->info.gfx;
+    return 0;
+}
+
+int hpccg_dev_malloc(hpccg_ctx *c, void **ptr, int64_t bytes)
+{
+    HPCCG_CHECK_ARG(c && ptr && bytes >= 0, "bad argument");
+    HPCCG_HIP_TRY(hipSetDevice(c->device));
+    if (hipMalloc(ptr, std::max<int64_t>(bytes, 1)) != hipSuccess) {
+        set_error("hpccg_dev_malloc: %lld bytes failed", (long long)bytes);
+        return HPCCG_ERR_NOMEM;
+    }
+    return 0;
+}
+
+int hpccg_dev_free(hpccg_ctx *c, void *ptr)
+{
+    HPCCG_CHECK_ARG(c, "null context");
+    HPCCG_HIP_TRY(hipStreamSynchronize(c->stream));
+    HPCCG_HIP_TRY(hipFree(ptr));
+    return 0;
+}
+
+int hpccg_memcpy_h2d(hpccg_ctx *c, void *dst, const void *src, int64_t bytes)
+{
+    HPCCG_CHECK_ARG(c && bytes >= 0, "bad argument");
+    HPCCG_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    HPCCG_HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int hpccg_memcpy_d2h(hpccg_ctx *c, void *dst, const void *src, int64_t bytes)
+{
+    HPCCG_CHECK_ARG(c && bytes >= 0, "bad argument");
+    HPCCG_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    HPCCG_HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int hpccg_memcpy_d2d(hpccg_ctx *c, void *dst, const void *src, int64_t bytes)
+{
+    HPCCG_CHECK_ARG(c && bytes >= 0, "bad argument");
+    HPCCG_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
+    HPCCG_HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+static int finish_matrix_setup(hpccg_ctx *c, const int32_t *rowindex_host)
+{
+    // per-row and per-tile maxima decide the SpMV path and are reported to callers
+    const int n = c->A.nrow;
+    int32_t mr = 0, mt = 0;
+    for (int i = 0; i < n; ++i) mr = std::max(mr, rowindex_host[i + 1] - rowindex_host[i]);
+    for (int t = 0; t * kTileRows < n; ++t) {
+        const int r1 = std::min(n, (t + 1) * kTileRows);
+        mt = std::max(mt, rowindex_host[r1] - rowindex_host[t * kTileRows]);
+    }
+    c->A.max_row_nnz = mr;
+    c->A.max_tile_nnz = mt;
+    for (auto &g : c->graphs) hipGraphExecDestroy(g.second);
+    c->graphs.clear();
+    return 0;
+}
+
+static void free_matrix(hpccg_ctx *c)
+{
+    hipFree(c->A.rowindex);
+    hipFree(c->A.colindex);
+    hipFree(c->A.values);
+    hipFree(c->A.values16);
+    c->A = Matrix{};
+}
+
+int hpccg_matrix_upload(hpccg_ctx *c, int32_t nrow, int32_t ncol, const int32_t *rowindex,
+                        const int32_t *colindex, const double *values)
+{
+    HPCCG_CHECK_ARG(c && rowindex && nrow >= 0, "bad argument");
+    HPCCG_CHECK_ARG(ncol == nrow, "single-rank matrix: ncol must equal nrow");
+    HPCCG_CHECK_ARG(nrow == 0 || rowindex[0] == 1, "rowindex must be 1-based (rowindex[0] == 1)");
+    const int64_t nnz = (int64_t)rowindex[nrow] - 1;
+    HPCCG_CHECK_ARG(nnz >= 0, "rowindex[nrow] must be >= 1");
+    HPCCG_CHECK_ARG(nnz == 0 || (colindex && values), "null colindex/values");
+    for (int i = 0; i < nrow; ++i)
+        HPCCG_CHECK_ARG(rowindex[i + 1] >= rowindex[i], "rowindex must be non-decreasing");
+    for (int64_t k = 0; k < nnz; ++k)
+        HPCCG_CHECK_ARG(colindex[k] >= 1 && colindex[k] <= ncol, "colindex out of range 1..ncol");
+    HPCCG_HIP_TRY(hipSetDevice(c->device));
+    HPCCG_HIP_TRY(hipStreamSynchronize(c->stream));
+    free_matrix(c);
+    c->A.nrow = nrow;
+    c->A.ncol = ncol;
+    c->A.nnz = nnz;
+    if (hipMalloc(&c->A.rowindex, (nrow + 1) * sizeof(int32_t)) != hipSuccess ||
+        hipMalloc(&c->A.colindex, std::max<int64_t>(nnz, 1) * sizeof(int32_t)) != hipSuccess ||
+        hipMalloc(&c->A.values, std::max<int64_t>(nnz, 1) * sizeof(double)) != hipSuccess) {
+        free_matrix(c);
+        set_error("hpccg_matrix_upload: device allocation failed (nnz=%lld)", (long long)nnz);
+        return HPCCG_ERR_NOMEM;
+    }
+    HPCCG_HIP_TRY(hipMemcpyAsync(c->A.rowindex, rowindex, (nrow + 1) * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    if (nnz) {
+        HPCCG_HIP_TRY(hipMemcpyAsync(c->A.colindex, colindex, nnz * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+        HPCCG_HIP_TRY(hipMemcpyAsync(c->A.values, values, nnz * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    }
+    HPCCG_HIP_TRY(hipStreamSynchronize(c->stream));
+    return finish_matrix_setup(c, rowindex);
+}
+
+int hpccg_generate_matrix(hpccg_ctx *c, int32_t nx, int32_t ny, int32_t nz, int32_t *rowindex,
+                          int32_t *colindex, double *values, double *x, double *b, double *xexact,
+                          int64_t *nnz_out)
+{
+    HPCCG_CHECK_ARG(c && nx >= 1 && ny >= 1 && nz >= 1, "grid extents must be >= 1");
+    const int64_t N = (int64_t)nx * ny * nz;
+    const int64_t nnz = (int64_t)(3 * nx - 2) * (3 * ny - 2) * (3 * nz - 2);
+    HPCCG_CHECK_ARG(N < INT32_MAX && nnz + 1 < INT32_MAX, "grid too large for int32 CSR");
+    HPCCG_HIP_TRY(hipSetDevice(c->device));
+    HPCCG_HIP_TRY(hipStreamSynchronize(c->stream));
+    free_matrix(c);
+    c->A.nrow = c->A.ncol = (int32_t)N;
+    c->A.nnz = nnz;
+    c->A.grid[0] = nx;
+    c->A.grid[1] = ny;
+    c->A.grid[2] = nz;
+    double *dx = nullptr, *db = nullptr, *dxe = nullptr;
+    if (hipMalloc(&c->A.rowindex, (N + 1) * sizeof(int32_t)) != hipSuccess ||
+        hipMalloc(&c->A.colindex, nnz * sizeof(int32_t)) != hipSuccess ||
+        hipMalloc(&c->A.values, nnz * sizeof(double)) != hipSuccess ||
+        hipMalloc(&dx, N * sizeof(double)) != hipSuccess || hipMalloc(&db, N * sizeof(double)) != hipSuccess ||
+        hipMalloc(&dxe, N * sizeof(double)) != hipSuccess) {
+        hipFree(dx);
+        hipFree(db);
+        hipFree(dxe);
+        free_matrix(c);
+        set_error("hpccg_generate_matrix: device allocation failed");
+        return HPCCG_ERR_NOMEM;
+    }
+    hipLaunchKernelGGL(k_generate, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, c->stream, nx, ny, nz,
+                       c->A.rowindex, c->A.colindex, c->A.values, dx, db, dxe);
+    hipError_t e = hipGetLastError();
+    std::vector<int32_t> rowh(N + 1);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(rowh.data(), c->A.rowindex, (N + 1) * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && colindex)
+        e = hipMemcpyAsync(colindex, c->A.colindex, nnz * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && values)
+        e = hipMemcpyAsync(values, c->A.values, nnz * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && x) e = hipMemcpyAsync(x, dx, N * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && b) e = hipMemcpyAsync(b, db, N * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && xexact) e = hipMemcpyAsync(xexact, dxe, N * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipFree(dx);
+    hipFree(db);
+    hipFree(dxe);
+    if (e != hipSuccess) {
+        set_error("hpccg_generate_matrix: %s", hipGetErrorString(e));
+        return HPCCG_ERR_HIP;
+    }
+    if (rowindex) memcpy(rowindex, rowh.data(), (N + 1) * sizeof(int32_t));
+    if (nnz_out) *nnz_out = nnz;
+    return finish_matrix_setup(c, rowh.data());
+}
+
+int hpccg_matrix_info(hpccg_ctx *c, int32_t *nrow, int64_t *nnz, int32_t *max_row_nnz)
+{
+    HPCCG_CHECK_ARG(c, "null context");
+    if (nrow) *nrow = c->A.nrow;
+    if (nnz) *nnz = c->A.nnz;
+    if (max_row_nnz) *max_row_nnz = c->A.max_row_nnz;
+    return 0;
+}
+
+// ---- device-array kernels ------------------------------------------------------------------
+int hpccg_spmv_dev(hpccg_ctx *c, const double *x, double *y)
+{
+    HPCCG_CHECK_ARG(c && x && y, "null argument");
+    HPCCG_TRY(matrix_required(c));
+    HPCCG_TRY(launch_spmv(c, x, y));
+    HPCCG_HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int hpccg_ddot_dev(hpccg_ctx *c, int32_t n, const double *x, const double *y, double *r)
+{
+    HPCCG_CHECK_ARG(c && x && y && r && n >= 0, "bad argument");
+    if (n == 0) {
+        *r = 0.0;
+        return 0;
+    }
+    HPCCG_TRY(launch_dot(c, n, x, y, c->scalar));
+    HPCCG_HIP_TRY(hipMemcpyAsync(c->scalar_h, c->scalar, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HPCCG_HIP_TRY(hipStreamSynchronize(c->stream));
+    *r = *c->scalar_h;
+    return 0;
+}
+
+static int vec_launch_grid(hpccg_ctx *c, int n) { return std::max(1, std::min((n + 255) / 256, c->info.num_cu * 8)); }
+
+int hpccg_waxpby_dev(hpccg_ctx *c, int32_t n, double alpha, const double *x, double beta, const double *y, double *w)
+{
+    HPCCG_CHECK_ARG(c && n >= 0 && (n == 0 || (x && y && w)), "bad argument");
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_waxpby, dim3(vec_launch_grid(c, n)), dim3(256), 0, c->stream, n, alpha, x, beta, y, w);
+    HPCCG_HIP_TRY(hipGetLastError());
+    HPCCG_HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int hpccg_wwpby_dev(hpccg_ctx *c, int32_t n, double beta, const double *y, double *w)
+{
+    HPCCG_CHECK_ARG(c && n >= 0 && (n == 0 || (y && w)), "bad argument");
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_wwpby, dim3(vec_launch_grid(c, n)), dim3(256), 0, c->stream, n, beta, y, w);
+    HPCCG_HIP_TRY(hipGetLastError());
+    HPCCG_HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int hpccg_wxbw_dev(hpccg_ctx *c, int32_t n, const double *x, double beta, double *w)
+{
+    HPCCG_CHECK_ARG(c && n >= 0 && (n == 0 || (x && w)), "bad argument");
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_wxbw, dim3(vec_launch_grid(c, n)), dim3(256), 0, c->stream, n, x, beta, w);
+    HPCCG_HIP_TRY(hipGetLastError());
+    HPCCG_HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int hpccg_cg_dev(hpccg_ctx *c, const double *b, double *x, int32_t max_iter, double tolerance, int32_t *niters,
+                 double *normr, double *hist, double *elapsed_ms)
+{
+    HPCCG_CHECK_ARG(c && b && x, "null argument");
+    HPCCG_HIP_TRY(hipSetDevice(c->device));
+    return cg_device(c, b, x, max_iter, tolerance, niters, normr, hist, elapsed_ms);
+}
+
+// ---- host-array drop-ins ---------------------------------------------------------------------
+int hpccg_spmv(hpccg_ctx *c, const double *x, double *y)
+{
+    HPCCG_CHECK_ARG(c && x && y, "null argument");
+    HPCCG_TRY(matrix_required(c));
+    const int64_t n = c->A.nrow;
+    HPCCG_TRY(ensure_stage(c, n));
+    HPCCG_HIP_TRY(hipMemcpyAsync(c->s0, x, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HPCCG_TRY(launch_spmv(c, c->s0, c->s1));
+    HPCCG_HIP_TRY(hipMemcpyAsync(y, c->s1, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HPCCG_HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int hpccg_ddot(hpccg_ctx *c, int32_t n, const double *x, const double *y, double *r)
+{
+    HPCCG_CHECK_ARG(c && r && n >= 0 && (n == 0 || (x && y)), "bad argument");
+    if (n == 0) {
+        *r = 0.0;
+        return 0;
+    }
+    HPCCG_TRY(ensure_stage(c, n));
+    HPCCG_HIP_TRY(hipMemcpyAsync(c->s0, x, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HPCCG_HIP_TRY(hipMemcpyAsync(c->s1, y, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    return hpccg_ddot_dev(c, n, c->s0, c->s1, r);
+}
+
+int hpccg_ddot1(hpccg_ctx *c, int32_t n, const double *x, double *r)
+{
+    HPCCG_CHECK_ARG(c && r && n >= 0 && (n == 0 || x), "bad argument");
+    if (n == 0) {
+        *r = 0.0;
+        return 0;
+    }
+    HPCCG_TRY(ensure_stage(c, n));
+    HPCCG_HIP_TRY(hipMemcpyAsync(c->s0, x, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    return hpccg_ddot_dev(c, n, c->s0, c->s0, r);
+}
+
+int hpccg_waxpby(hpccg_ctx *c, int32_t n, double alpha, const double *x, double beta, const double *y, double *w)
+{
+    HPCCG_CHECK_ARG(c && n >= 0 && (n == 0 || (x && y && w)), "bad argument");
+    if (n == 0) return 0;
+    HPCCG_TRY(ensure_stage(c, n));
+    HPCCG_HIP_TRY(hipMemcpyAsync(c->s0, x, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HPCCG_HIP_TRY(hipMemcpyAsync(c->s1, y, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HPCCG_TRY(hpccg_waxpby_dev(c, n, alpha, c->s0, beta, c->s1, c->s2));
+    return hpccg_memcpy_d2h(c, w, c->s2, n * sizeof(double));
+}
+
+int hpccg_wwpby(hpccg_ctx *c, int32_t n, double beta, const double *y, double *w)
+{
+    HPCCG_CHECK_ARG(c && n >= 0 && (n == 0 || (y && w)), "bad argument");
+    if (n == 0) return 0;
+    HPCCG_TRY(ensure_stage(c, n));
+    HPCCG_HIP_TRY(hipMemcpyAsync(c->s0, y, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HPCCG_HIP_TRY(hipMemcpyAsync(c->s2, w, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HPCCG_TRY(hpccg_wwpby_dev(c, n, beta, c->s0, c->s2));
+    return hpccg_memcpy_d2h(c, w, c->s2, n * sizeof(double));
+}
+
+int hpccg_wxbw(hpccg_ctx *c, int32_t n, const double *x, double beta, double *w)
+{
+    HPCCG_CHECK_ARG(c && n >= 0 && (n == 0 || (x && w)), "bad argument");
+    if (n == 0) return 0;
+    HPCCG_TRY(ensure_stage(c, n));
+    HPCCG_HIP_TRY(hipMemcpyAsync(c->s0, x, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HPCCG_HIP_TRY(hipMemcpyAsync(c->s2, w, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HPCCG_TRY(hpccg_wxbw_dev(c, n, c->s0, beta, c->s2));
+    return hpccg_memcpy_d2h(c, w, c->s2, n * sizeof(double));
+}
+
+int hpccg_vcopy(hpccg_ctx *c, int32_t n, const double *x, double *y)
+{
+    HPCCG_CHECK_ARG(c && n >= 0 && (n == 0 || (x && y)), "bad argument");
+    if (n == 0) return 0;
+    HPCCG_TRY(ensure_stage(c, n));
+    HPCCG_HIP_TRY(hipMemcpyAsync(c->s0, x, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HPCCG_HIP_TRY(hipMemcpyAsync(c->s1, c->s0, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    return hpccg_memcpy_d2h(c, y, c->s1, n * sizeof(double));
+}
+
+int hpccg_cg(hpccg_ctx *c, const double *b, double *x, int32_t max_iter, double tolerance, int32_t *niters,
+             double *normr, double *hist)
+{
+    HPCCG_CHECK_ARG(c && b && x, "null argument");
+    HPCCG_TRY(matrix_required(c));
+    HPCCG_HIP_TRY(hipSetDevice(c->device));
+    const int64_t n = c->A.nrow;
+    HPCCG_TRY(ensure_stage(c, n));
+    HPCCG_HIP_TRY(hipMemcpyAsync(c->s0, b, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HPCCG_HIP_TRY(hipMemcpyAsync(c->s1, x, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    const int st = cg_device(c, c->s0, c->s1, max_iter, tolerance, niters, normr, hist, nullptr);
+    if (st < 0) return st;
+    HPCCG_TRY(hpccg_memcpy_d2h(c, x, c->s1, n * sizeof(double)));
+    return st;
+}
+
+// ---- measurement ----------------------------------------------------------------------------
+int hpccg_time_spmv(hpccg_ctx *c, const double *x, double *y, int32_t reps, double *avg_ms)
+{
+    // Times the SpMV kernel exactly as the CG loop launches it (k_cg_spmv with the p.Ap tree
+    // epilogue), driven by a private control block that never stops.
+    HPCCG_CHECK_ARG(c && x && y && reps > 0 && avg_ms, "bad argument");
+    HPCCG_TRY(matrix_required(c));
+    HPCCG_TRY(ensure_hist(c, 2));
+    CgState init{};
+    init.k = 1;
+    init.stop_at = INT32_MAX;
+    init.max_iter = INT32_MAX - 1;
+    *c->state_h = init;
+    HPCCG_HIP_TRY(hipMemcpyAsync(c->state, c->state_h, sizeof(CgState), hipMemcpyHostToDevice, c->stream));
+    const double one = 1.0;
+    HPCCG_HIP_TRY(hipMemcpyAsync(c->rr_hist, &one, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    const int n = c->A.nrow, nt = ntiles_of(c), G = spmv_grid(c, nt);
+    auto launch = [&]() {
+        hipLaunchKernelGGL((k_cg_spmv<kTileRows, true>), dim3(G), dim3(kTileRows), 0, c->stream, c->A.rowindex,
+                           c->A.colindex, c->A.values, x, y, n, nt, c->state, c->rr_hist, c->partials);
+    };
+    launch();  // warm
+    HPCCG_HIP_TRY(hipEventRecord(c->ev0, c->stream));
+    for (int i = 0; i < reps; ++i) launch();
+    HPCCG_HIP_TRY(hipGetLastError());
+    HPCCG_HIP_TRY(hipEventRecord(c->ev1, c->stream));
+    HPCCG_HIP_TRY(hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    HPCCG_HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    *avg_ms = ms / reps;
+    return 0;
+}
+
+int hpccg_time_copy(hpccg_ctx *c, void *dst, const void *src, int64_t bytes, int32_t reps, double *avg_ms)
+{
+    HPCCG_CHECK_ARG(c && dst && src && reps > 0 && avg_ms && bytes % 16 == 0, "bad argument (bytes % 16 == 0)");
+    const int64_t n16 = bytes / 16;
+    const int G = c->info.num_cu * 8;
+    hipLaunchKernelGGL(k_copy16, dim3(G), dim3(256), 0, c->stream, (const u32x4 *)src, (u32x4 *)dst, n16);
+    HPCCG_HIP_TRY(hipEventRecord(c->ev0, c->stream));
+    for (int i = 0; i < reps; ++i)
+        hipLaunchKernelGGL(k_copy16, dim3(G), dim3(256), 0, c->stream, (const u32x4 *)src, (u32x4 *)dst, n16);
+    HPCCG_HIP_TRY(hipGetLastError());
+    HPCCG_HIP_TRY(hipEventRecord(c->ev1, c->stream));
+    HPCCG_HIP_TRY(hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    HPCCG_HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    *avg_ms = ms / reps;
+    return 0;
+}
+
+}  // extern "C"

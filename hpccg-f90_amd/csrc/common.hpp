@@ -1,0 +1,115 @@
+// Shared device/host helpers for libhpccg_hip (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "hpccg_hip.h"
+
+namespace hpccg {
+
+// ------------------------------------------------------------------------------------------
+// error reporting: thread-local message + negative status codes
+void set_error(const char *fmt, ...);
+
+#define HPCCG_HIP_TRY(call)                                                                    \
+    do {                                                                                       \
+        hipError_t _e = (call);                                                                \
+        if (_e != hipSuccess) {                                                                \
+            ::hpccg::set_error("%s:%d %s -> %s", __FILE__, __LINE__, #call, hipGetErrorString(_e)); \
+            return HPCCG_ERR_HIP;                                                              \
+        }                                                                                      \
+    } while (0)
+
+#define HPCCG_CHECK_ARG(cond, msg)                                                             \
+    do {                                                                                       \
+        if (!(cond)) {                                                                         \
+            ::hpccg::set_error("%s: %s", __func__, msg);                                       \
+            return HPCCG_ERR_ARG;                                                              \
+        }                                                                                      \
+    } while (0)
+
+#define HPCCG_TRY(call)                                                                        \
+    do {                                                                                       \
+        int _s = (call);                                                                       \
+        if (_s < 0) return _s;                                                                 \
+    } while (0)
+
+constexpr int kWave = 64;
+constexpr int kNumXcd = 8;  // MI355X: 8 XCDs, each with its own L2; workgroup i -> XCD i % 8
+
+// Workgroups are dealt to XCDs round-robin.  Renumber them so that XCD x owns the contiguous
+// logical range [x*G/8, (x+1)*G/8): neighbouring row tiles (which share x-vector planes)
+// then run on the same XCD and hit in that XCD's L2.  G must be a multiple of 8.
+__device__ __forceinline__ int xcd_logical_block(int b, int G)
+{
+    return (b % kNumXcd) * (G / kNumXcd) + b / kNumXcd;
+}
+
+// ------------------------------------------------------------------------------------------
+// reductions (fixed shape => deterministic)
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+// Block-wide sum; every thread gets the total.  `red` needs blockDim/64 doubles of LDS.
+template <int THREADS>
+__device__ __forceinline__ double block_sum(double v, double *red)
+{
+    constexpr int W = THREADS / kWave;
+    v = wave_sum(v);
+    const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+    if (W == 1) return v;
+    __syncthreads();
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    double t = 0.0;
+#pragma unroll
+    for (int i = 0; i < W; ++i) t += red[i];
+    return t;
+}
+
+__device__ __forceinline__ double load_agent(const double *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Single-pass grid reduction.  Every block calls this with its block total (valid in thread
+// 0).  Returns true in exactly one block -- the last to arrive -- whose threads then all hold
+// the grid total in *total, summed in a fixed order over partials[0..G) (independent of
+// arrival order).  The counter is reset for the next use.
+template <int THREADS>
+__device__ __forceinline__ bool grid_sum(double block_total, double *partials, unsigned *counter,
+                                         int G, int logical_b, double *red, double *total)
+{
+    __shared__ int is_last;
+    if (threadIdx.x == 0) {
+        partials[logical_b] = block_total;
+        __threadfence();
+        const unsigned old = atomicAdd(counter, 1u);
+        is_last = (old == (unsigned)G - 1u);
+    }
+    __syncthreads();
+    if (!is_last) return false;
+    __threadfence();
+    double s = 0.0;
+    for (int i = threadIdx.x; i < G; i += THREADS) s += load_agent(partials + i);
+    s = block_sum<THREADS>(s, red);
+    *total = s;
+    if (threadIdx.x == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return true;
+}
+
+// Device-wide launch facts, queried once per context.
+struct DeviceInfo {
+    int num_cu = 256;
+    int gfx = 950;
+};
+
+}  // namespace hpccg

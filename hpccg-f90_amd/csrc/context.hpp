@@ -1,0 +1,76 @@
+// Context object behind the opaque hpccg_ctx handle.
+#pragma once
+
+#include <hip/hip_fp16.h>
+
+#include <map>
+#include <tuple>
+
+#include "common.hpp"
+
+namespace hpccg {
+
+// Device-resident CG control block.  The iteration loop reads and updates it entirely on
+// the device, so a captured graph needs no host round trip per iteration.
+struct CgState {
+    double alpha;      // alpha of the last iteration that computed p.Ap
+    double beta;       // beta for the next p update
+    double tol;        // absolute tolerance on the recursive residual norm
+    int k;             // iteration to run next (1-based, numerics.F90:108)
+    int stop_at;       // first iteration that does not run (max_iter+1 or lagged test)
+    int max_iter;
+    int breakdown;     // p.Ap was zero or non-finite
+    unsigned counter[4];
+    int pad[4];
+};
+
+struct Matrix {
+    int32_t nrow = 0, ncol = 0;
+    int64_t nnz = 0;
+    int32_t max_row_nnz = 0;
+    int32_t max_tile_nnz = 0;  // max nnz over row tiles of kTileRows rows
+    int32_t *rowindex = nullptr;  // 1-based, nrow+1
+    int32_t *colindex = nullptr;  // 1-based, nnz
+    double *values = nullptr;
+    __half *values16 = nullptr;   // FP16 copy for the mixed-precision path (lazy)
+    int32_t grid[3] = {0, 0, 0};  // nx, ny, nz when built by hpccg_generate_matrix
+};
+
+struct GraphKey {
+    const void *b, *x;
+    int mode, chunk;
+    bool operator<(const GraphKey &o) const
+    {
+        return std::tie(b, x, mode, chunk) < std::tie(o.b, o.x, o.mode, o.chunk);
+    }
+};
+
+}  // namespace hpccg
+
+struct hpccg_ctx {
+    int device = 0;
+    hpccg::DeviceInfo info;
+    hipStream_t stream = nullptr;
+    int dot_mode = HPCCG_DOT_TREE;
+    hpccg::Matrix A;
+
+    // CG workspace (sized to A.nrow)
+    int64_t ws_n = 0;
+    double *r = nullptr, *p = nullptr, *Ap = nullptr;
+    hpccg::CgState *state = nullptr;    // device
+    hpccg::CgState *state_h = nullptr;  // pinned host mirror
+    double *rr_hist = nullptr;          // device, hist_cap doubles
+    int hist_cap = 0;
+    double *partials = nullptr;         // device, kMaxGrid doubles
+    double *scalar = nullptr;           // device result slot for standalone dots
+    double *scalar_h = nullptr;         // pinned host
+
+    // staging for the host-array API
+    int64_t stage_n = 0;
+    double *s0 = nullptr, *s1 = nullptr, *s2 = nullptr;
+
+    std::map<hpccg::GraphKey, hipGraphExec_t> graphs;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    void *ext[8] = {};  // owned by the mixed / batched / learned paths
+};

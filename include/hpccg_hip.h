@@ -1,0 +1,139 @@
+/*
+ * libhpccg_hip -- MI355X (gfx950) conjugate-gradient path of HPCCG-F90, C ABI.
+ *
+ * Drop-in boundary for the reference's Fortran procedures (RRZE-HPC/HPCCG-F90, MPI/ variant):
+ * the Fortran modules in hpccg-f90_amd/fortran keep the reference's module names, procedure
+ * names and argument lists and forward to the functions below through ISO_C_BINDING.
+ *
+ * Conventions
+ *   - CSR arrays are passed exactly as generate_matrix builds them: 1-based rowindex[nrow+1]
+ *     (rowindex[0] == 1), 1-based colindex[nnz], FP64 values[nnz]  (SparseMatrix.F90:44-64).
+ *   - All counts are int32_t, as in the reference (default INTEGER).
+ *   - Functions without a _dev suffix take HOST pointers (Fortran arrays) and are synchronous.
+ *     Functions with a _dev suffix take DEVICE pointers (from hpccg_dev_malloc) and are
+ *     synchronous too unless stated otherwise.
+ *   - Every function returns 0 on success or a negative HPCCG_ERR_* code; hpccg_last_error()
+ *     returns the message of the most recent failure on the calling thread.
+ *   - One context = one device + one HIP stream; calls on one context are not re-entrant.
+ */
+#ifndef HPCCG_HIP_H
+#define HPCCG_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hpccg_ctx hpccg_ctx;
+
+enum {
+    HPCCG_OK = 0,
+    HPCCG_ERR_ARG = -1,       /* invalid argument (size mismatch, null pointer, bad mode)     */
+    HPCCG_ERR_HIP = -2,       /* a HIP runtime call failed                                     */
+    HPCCG_ERR_STATE = -3,     /* no matrix uploaded / wrong call order                          */
+    HPCCG_ERR_NOMEM = -4,     /* device allocation failed                                       */
+    HPCCG_WARN_BREAKDOWN = 1  /* cg finished but p.Ap was zero or non-finite in some iteration  */
+};
+
+/* Dot-product summation order used by the FP64 path.
+ *   SEQUENTIAL: one left-to-right FP64 chain from 0.0, exactly the order of the reference's
+ *               serial ddot/ddot1 (vector_operations.F90:54-59,86-91); cg then reproduces the
+ *               reference's iterates bit for bit.  Slow (a serial chain on one wavefront).
+ *   TREE:       fixed-shape wavefront/workgroup/grid tree (deterministic, run-to-run identical);
+ *               the default fast path. */
+enum { HPCCG_DOT_SEQUENTIAL = 0, HPCCG_DOT_TREE = 1 };
+
+/* ---- lifecycle ------------------------------------------------------------------------- */
+int hpccg_create(hpccg_ctx **ctx, int device);
+int hpccg_destroy(hpccg_ctx *ctx);
+const char *hpccg_last_error(void);
+int hpccg_set_dot_mode(hpccg_ctx *ctx, int mode);
+int hpccg_get_dot_mode(hpccg_ctx *ctx, int *mode);
+/* Device facts used for launch shapes: CU count, XCD count, gfx arch number (e.g. 950). */
+int hpccg_device_info(hpccg_ctx *ctx, int *num_cu, int *num_xcd, int *gfx);
+
+/* ---- device memory (for callers that keep vectors resident) ----------------------------- */
+int hpccg_dev_malloc(hpccg_ctx *ctx, void **ptr, int64_t bytes);
+int hpccg_dev_free(hpccg_ctx *ctx, void *ptr);
+int hpccg_memcpy_h2d(hpccg_ctx *ctx, void *dst, const void *src, int64_t bytes);
+int hpccg_memcpy_d2h(hpccg_ctx *ctx, void *dst, const void *src, int64_t bytes);
+int hpccg_memcpy_d2d(hpccg_ctx *ctx, void *dst, const void *src, int64_t bytes);
+
+/* ---- matrix ------------------------------------------------------------------------------ */
+/* Upload the CSR built by generate_matrix (SparseMatrix.F90:68-185); replaces any previous
+ * matrix.  ncol == nrow on a single rank (SparseMatrix.F90:183).  The arrays are copied. */
+int hpccg_matrix_upload(hpccg_ctx *ctx, int32_t nrow, int32_t ncol, const int32_t *rowindex,
+                        const int32_t *colindex, const double *values);
+/* Build generate_matrix's CSR for an nx*ny*nz grid directly in device memory (same bytes as
+ * the host build; SparseMatrix.F90:138-175).  Any of the host output pointers may be NULL;
+ * non-NULL ones receive a copy (rowindex[N+1], colindex/values[nnz], x/b/xexact[N]).
+ * *nnz_out receives the true nnz. */
+int hpccg_generate_matrix(hpccg_ctx *ctx, int32_t nx, int32_t ny, int32_t nz, int32_t *rowindex,
+                          int32_t *colindex, double *values, double *x, double *b, double *xexact,
+                          int64_t *nnz_out);
+int hpccg_matrix_info(hpccg_ctx *ctx, int32_t *nrow, int64_t *nnz, int32_t *max_row_nnz);
+
+/* ---- kernels, host arrays (drop-ins for the reference procedures) ---------------------- */
+/* spmv(A, x, y)             SparseMatrix.F90:213-232 */
+int hpccg_spmv(hpccg_ctx *ctx, const double *x, double *y);
+/* ddot(n, x, y, r)          vector_operations.F90:40-71 */
+int hpccg_ddot(hpccg_ctx *ctx, int32_t n, const double *x, const double *y, double *r);
+/* ddot1(n, x, r)            vector_operations.F90:73-103 */
+int hpccg_ddot1(hpccg_ctx *ctx, int32_t n, const double *x, double *r);
+/* waxpby(n, alpha, x, beta, y, w): w = alpha*x + beta*y     vector_operations.F90:105-121 */
+int hpccg_waxpby(hpccg_ctx *ctx, int32_t n, double alpha, const double *x, double beta,
+                 const double *y, double *w);
+/* wwpby(n, beta, y, w): w = w + beta*y                      vector_operations.F90:123-137 */
+int hpccg_wwpby(hpccg_ctx *ctx, int32_t n, double beta, const double *y, double *w);
+/* wxbw(n, x, beta, w): w = x + beta*w                       vector_operations.F90:139-153 */
+int hpccg_wxbw(hpccg_ctx *ctx, int32_t n, const double *x, double beta, double *w);
+/* vcopy(n, x, y): y = x                                     vector_operations.F90:155-169 */
+int hpccg_vcopy(hpccg_ctx *ctx, int32_t n, const double *x, double *y);
+
+/* cg(A, b, x, max_iter, tolerance, niters, normr)           numerics.F90:41-162
+ * Unpreconditioned CG from the x passed in (updated in place), same operation order, same
+ * lagging convergence test (exit at the top of iteration k when ||r_{k-2}|| < tolerance),
+ * niters = last iteration run, normr = ||r_{niters-1}||.  hist (may be NULL) receives ||r_j||
+ * for j = 0..max_iter (max_iter+1 doubles; entries never formed are -1.0; hist[max_iter] is
+ * ||r_max_iter||, which the reference computes but never norms).  Returns HPCCG_WARN_BREAKDOWN
+ * if p.Ap was zero or non-finite in some iteration (the reference continues with NaNs, and so
+ * does this). */
+int hpccg_cg(hpccg_ctx *ctx, const double *b, double *x, int32_t max_iter, double tolerance,
+             int32_t *niters, double *normr, double *hist);
+
+/* ---- kernels, device arrays --------------------------------------------------------------- */
+int hpccg_spmv_dev(hpccg_ctx *ctx, const double *x, double *y);
+int hpccg_ddot_dev(hpccg_ctx *ctx, int32_t n, const double *x, const double *y, double *r);
+int hpccg_waxpby_dev(hpccg_ctx *ctx, int32_t n, double alpha, const double *x, double beta,
+                     const double *y, double *w);
+int hpccg_wwpby_dev(hpccg_ctx *ctx, int32_t n, double beta, const double *y, double *w);
+int hpccg_wxbw_dev(hpccg_ctx *ctx, int32_t n, const double *x, double beta, double *w);
+/* cg with b and x resident on the device.  *elapsed_ms (may be NULL) receives the device time
+ * of the whole solve -- Prepare, iteration loop and final x update (HIP events on the
+ * context's stream; the reference's "Total" timer, numerics.F90:106, covers the loop only). */
+int hpccg_cg_dev(hpccg_ctx *ctx, const double *b, double *x, int32_t max_iter, double tolerance,
+                 int32_t *niters, double *normr, double *hist, double *elapsed_ms);
+
+/* ---- seeded synthetic inputs (no reference counterpart: the reference hard-codes x* = 1,
+ * SparseMatrix.F90:170-172).  Host arrays; deterministic SplitMix64 streams. --------------- */
+/* x* = sum of 8 random low-frequency 3-D sines with random phases + U(-0.05,0.05) noise */
+int hpccg_xstar_sines(int32_t nx, int32_t ny, int32_t nz, uint64_t seed, double *xstar);
+/* x*_i ~ U(-1,1) */
+int hpccg_xstar_uniform(int64_t n, uint64_t seed, double *xstar);
+/* 3*count extents ~ U{lo..hi}: (nx,ny,nz) of each batched system */
+int hpccg_batch_dims(int32_t count, int32_t lo, int32_t hi, uint64_t seed, int32_t *dims);
+
+/* ---- measurement -------------------------------------------------------------------------- */
+/* Run `reps` back-to-back FP64 SpMVs y = A x on device arrays -- the kernel the CG loop
+ * launches, with its p.Ap epilogue -- and return the mean kernel time in ms (HIP events on the
+ * context's stream, the stream the kernel is launched on). */
+int hpccg_time_spmv(hpccg_ctx *ctx, const double *x, double *y, int32_t reps, double *avg_ms);
+/* A plain device copy of `bytes` bytes, `reps` times: the HBM calibration kernel. */
+int hpccg_time_copy(hpccg_ctx *ctx, void *dst, const void *src, int64_t bytes, int32_t reps,
+                    double *avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
