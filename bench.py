@@ -126,8 +126,12 @@ def main():
         res = ctx.cg_dev(b, x, max_iter=2000, tolerance=REL_TOL * normb, want_hist=False)
         return res
 
-    solvers = {"fp64_cg": run_fp64}
-    headline = "fp64_cg"
+    def run_mixed():
+        x.upload(zeros)
+        return ctx.cg_mixed_dev(b, x, max_iter=2000, tolerance=REL_TOL * normb, replace_every=25,
+                                want_hist=False)
+
+    solvers = {"fp64_cg": run_fp64, "mixed_cg": run_mixed}
     results = {}
     for name, fn in solvers.items():
         for _ in range(args.warmup):
@@ -143,6 +147,12 @@ def main():
         results[name] = {"s_wall": sum(times) / len(times), "s_device": sum(dev_ms) / len(dev_ms) / 1e3,
                          "niters": res["niters"], "true_relres": rel, "xerr_max": xerr}
         log(f"{name}: {results[name]}")
+
+    # headline: the fastest method whose answer meets 1e-10 in the FP64 true residual
+    ok = [k for k, v in results.items() if v["true_relres"] <= REL_TOL]
+    headline = min(ok, key=lambda k: results[k]["s_wall"]) if ok else "fp64_cg"
+    for k in results:
+        results[k]["speedup_vs_fp64_cg"] = round(results["fp64_cg"]["s_wall"] / results[k]["s_wall"], 3)
 
     # roofline of the dominant kernel: the FP64 CSR SpMV as the CG loop launches it
     y = hpccg_amd.DeviceArray(ctx, n)
@@ -184,7 +194,7 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if headline == "fp64_cg" else "f16/f32/f64 (values/vectors/dots+x)",
         "data": "synthetic",
         "config": {"workload": f"cg-{n1}^3-sines-seed{SEED}-to-relres-1e-10", "grid": list(grid),
                    "solver": headline, "rows": n, "nnz": nnz},

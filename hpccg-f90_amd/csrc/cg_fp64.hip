@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "context.hpp"
+#include "spmv.hpp"
 
 namespace hpccg {
 
@@ -32,19 +33,8 @@ void set_error(const char *fmt, ...)
     va_end(ap);
 }
 
-constexpr int kTileRows = 128;        // rows per SpMV tile == threads per SpMV workgroup
-constexpr int kTileCap = kTileRows * 27;  // products staged in LDS per tile (27-point rows)
-constexpr int kVecThreads = 256;
-constexpr int kMaxGrid = 16384;
 
-// ========================================================================================
-// SpMV tile: rows [tile*ROWS, tile*ROWS+ROWS).  Phase 1 streams the tile's contiguous
-// (value, column) range with unit-stride (coalesced) loads, gathers x, and stages the rounded
-// products in LDS; phase 2 sums each row's products in ascending k from 0.0 -- the reference's
-// order (SparseMatrix.F90:225-229).  Tiles whose nnz exceed the LDS capacity (rows longer than
-// 27 on average; never the case for generate_matrix's CSR) take a direct per-row loop with the
-// same order.  Returns x[r] * y[r] (the p.Ap term when x is p).
-// ========================================================================================
+// FP64 SpMV tile (y = A x for this thread's row) returning x[r]*y[r], the p.Ap term when x = p.
 template <int ROWS>
 __device__ __forceinline__ double spmv_tile(const int32_t *__restrict__ rowptr,
                                             const int32_t *__restrict__ col,
@@ -52,59 +42,14 @@ __device__ __forceinline__ double spmv_tile(const int32_t *__restrict__ rowptr,
                                             const double *__restrict__ x, double *__restrict__ y,
                                             int nrow, int tile, double *lds)
 {
-    const int t = threadIdx.x;
-    const int r0 = tile * ROWS;
-    const int r = r0 + t;
-    const int rlast = min(r0 + ROWS, nrow);
-    const int base = rowptr[r0] - 1;
-    const int end = rowptr[rlast] - 1;
-    int ms = 0, me = 0;
-    if (r < nrow) {
-        ms = rowptr[r] - 1;
-        me = rowptr[r + 1] - 1;
-    }
-    double sum = 0.0;
-    if (end - base <= ROWS * 27) {
-        for (int k = base + t; k < end; k += 4 * ROWS) {
-            double v[4];
-            int c[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int kk = k + u * ROWS;
-                if (kk < end) {
-                    v[u] = __builtin_nontemporal_load(val + kk);
-                    c[u] = __builtin_nontemporal_load(col + kk);
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int kk = k + u * ROWS;
-                if (kk < end) lds[kk - base] = v[u] * x[c[u] - 1];
-            }
-        }
-        __syncthreads();
-        for (int j = ms; j < me; ++j) sum = sum + lds[j - base];
-        __syncthreads();
-    } else {
-        for (int j = ms; j < me; ++j) {
-            const double prod = val[j] * x[col[j] - 1];
-            sum = sum + prod;
-        }
-    }
-    if (r < nrow) {
+    bool valid;
+    int r;
+    const double sum = spmv_tile_rowsum<ROWS, double, double>(rowptr, col, val, x, nrow, tile, lds, valid, r);
+    if (valid) {
         y[r] = sum;
         return x[r] * sum;
     }
     return 0.0;
-}
-
-// Contiguous tile range of one (XCD-renumbered) workgroup.
-__device__ __forceinline__ void block_range(int ntiles, int &t0, int &t1, int &lb)
-{
-    const int G = gridDim.x;
-    lb = xcd_logical_block(blockIdx.x, G);
-    t0 = (int)(((int64_t)lb * ntiles) / G);
-    t1 = (int)(((int64_t)(lb + 1) * ntiles) / G);
 }
 
 // y = A x  (standalone spmv; no reduction)
@@ -190,15 +135,6 @@ __global__ __launch_bounds__(ROWS) void k_cg_prep_spmv(const int32_t *__restrict
     int t0, t1, lb;
     block_range(ntiles, t0, t1, lb);
     for (int tile = t0; tile < t1; ++tile) spmv_tile<ROWS>(rowptr, col, val, x, Ap, nrow, tile, lds);
-}
-
-// Element range [i0, i1) of one workgroup in a contiguous split of n elements.
-__device__ __forceinline__ void elem_range(int64_t n, int64_t &i0, int64_t &i1, int &lb)
-{
-    const int G = gridDim.x;
-    lb = xcd_logical_block(blockIdx.x, G);
-    i0 = (n * lb) / G;
-    i1 = (n * (lb + 1)) / G;
 }
 
 // Prepare: r = 1*b + (-1)*Ap and (TREE) r.r              numerics.F90:97-99
@@ -472,19 +408,8 @@ __global__ void k_generate(int nx, int ny, int nz, int32_t *__restrict__ rowinde
 // ============================================================================================
 // host side
 // ============================================================================================
-static int spmv_grid(const hpccg_ctx *c, int ntiles)
-{
-    // 4 workgroups of 128 threads per CU keep LDS (27.6 KB each) and waves balanced
-    int G = std::min(ntiles, c->info.num_cu * 8);
-    G = std::max(kNumXcd, (G / kNumXcd) * kNumXcd);
-    return G;
-}
-static int vec_grid(const hpccg_ctx *c, int64_t n)
-{
-    int64_t G = std::min<int64_t>((n + kVecThreads - 1) / kVecThreads, (int64_t)c->info.num_cu * 4);
-    G = std::max<int64_t>(kNumXcd, (G / kNumXcd) * kNumXcd);
-    return (int)G;
-}
+static int spmv_grid(const hpccg_ctx *c, int ntiles) { return spmv_grid_for(c->info.num_cu, ntiles); }
+static int vec_grid(const hpccg_ctx *c, int64_t n) { return vec_grid_for(c->info.num_cu, n); }
 
 static int ensure_workspace(hpccg_ctx *c, int64_t n)
 {
@@ -869,6 +794,8 @@ static int finish_matrix_setup(hpccg_ctx *c, const int32_t *rowindex_host)
     c->A.max_tile_nnz = mt;
     for (auto &g : c->graphs) hipGraphExecDestroy(g.second);
     c->graphs.clear();
+    for (auto &g : c->mix.graphs) hipGraphExecDestroy(g.second);
+    c->mix.graphs.clear();
     return 0;
 }
 

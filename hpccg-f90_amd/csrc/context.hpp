@@ -45,6 +45,17 @@ struct GraphKey {
     }
 };
 
+// Workspace of the mixed-precision path (FP16 values, FP32 vectors, FP64 dots and x).
+struct MixedWs {
+    int64_t n = 0;
+    float *r = nullptr, *p = nullptr, *Ap = nullptr;
+    void *state = nullptr;    // device MixState
+    void *state_h = nullptr;  // pinned host mirror
+    double *hist = nullptr;   // [2][hist_cap]: recursive and true r.r per iteration
+    int hist_cap = 0;
+    std::map<GraphKey, hipGraphExec_t> graphs;
+};
+
 }  // namespace hpccg
 
 struct hpccg_ctx {
@@ -72,5 +83,12 @@ struct hpccg_ctx {
     std::map<hpccg::GraphKey, hipGraphExec_t> graphs;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
-    void *ext[8] = {};  // owned by the mixed / batched / learned paths
+    hpccg::MixedWs mix;
+    void *ext[8] = {};  // owned by the batched / learned paths
 };
+
+namespace hpccg {
+void free_mixed(hpccg_ctx *c);
+void free_batched(hpccg_ctx *c);
+int launch_spmv(hpccg_ctx *c, const double *x, double *y);
+}  // namespace hpccg

@@ -3,6 +3,8 @@
 
 extern "C" void hpccg_free_ext(hpccg_ctx *c)
 {
+    hpccg::free_mixed(c);
+    hpccg::free_batched(c);
     for (void *&p : c->ext) {
         if (p) hipFree(p);
         p = nullptr;

@@ -53,6 +53,16 @@ _SIGS = {
     "hpccg_wwpby_dev": (ctypes.c_int, [P, I32, D, P, P]),
     "hpccg_wxbw_dev": (ctypes.c_int, [P, I32, P, D, P]),
     "hpccg_cg_dev": (ctypes.c_int, [P, P, P, I32, D, PI32, PD, P, PD]),
+    "hpccg_cg_mixed": (ctypes.c_int, [P, P, P, I32, D, I32, PI32, PD, P, P]),
+    "hpccg_cg_mixed_dev": (ctypes.c_int, [P, P, P, I32, D, I32, PI32, PD, P, P, PD]),
+    "hpccg_spmv_mixed_dev": (ctypes.c_int, [P, P, P]),
+    "hpccg_time_spmv_mixed": (ctypes.c_int, [P, P, P, I32, PD]),
+    "hpccg_batch_setup": (ctypes.c_int, [P, I32, P]),
+    "hpccg_batch_info": (ctypes.c_int, [P, PI32, PI64, PI64, P]),
+    "hpccg_batch_spmv_dev": (ctypes.c_int, [P, P, P]),
+    "hpccg_batch_variants": (ctypes.c_int, [PI32, P, P]),
+    "hpccg_batch_autotune": (ctypes.c_int, [P, P, P, I32, PI32, P]),
+    "hpccg_batch_solve_dev": (ctypes.c_int, [P, P, P, D, I32, I32, I32, P, P, P, PD]),
     "hpccg_xstar_sines": (ctypes.c_int, [I32, I32, I32, ctypes.c_uint64, P]),
     "hpccg_xstar_uniform": (ctypes.c_int, [I64, ctypes.c_uint64, P]),
     "hpccg_batch_dims": (ctypes.c_int, [I32, I32, I32, ctypes.c_uint64, P]),
@@ -290,6 +300,77 @@ class Context:
                                        ctypes.byref(nr), _ptr(hist) if want_hist else None,
                                        ctypes.byref(ms)), "cg_dev")
         return {"niters": nit.value, "normr": nr.value, "hist": hist, "status": st, "ms": ms.value}
+
+    def cg_mixed(self, b, x0=None, max_iter=1000, tolerance=0.0, replace_every=25):
+        """Mixed-precision CG on host arrays -> (x, niters, true_normr, hist_rec, hist_true, status)."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros(self.n) if x0 is None else np.array(x0, dtype=np.float64)
+        hr, ht = np.zeros(max_iter + 1), np.zeros(max_iter + 1)
+        nit, tn = ctypes.c_int32(), ctypes.c_double()
+        st = _check(lib().hpccg_cg_mixed(self.h, _ptr(b), _ptr(x), max_iter, tolerance, replace_every,
+                                         ctypes.byref(nit), ctypes.byref(tn), _ptr(hr), _ptr(ht)), "cg_mixed")
+        return x, nit.value, tn.value, hr, ht, st
+
+    def cg_mixed_dev(self, b: DeviceArray, x: DeviceArray, max_iter=1000, tolerance=0.0, replace_every=25,
+                     want_hist=True):
+        hr = np.zeros(max_iter + 1) if want_hist else None
+        ht = np.zeros(max_iter + 1) if want_hist else None
+        nit, tn, ms = ctypes.c_int32(), ctypes.c_double(), ctypes.c_double()
+        st = _check(lib().hpccg_cg_mixed_dev(self.h, b.ptr, x.ptr, max_iter, tolerance, replace_every,
+                                             ctypes.byref(nit), ctypes.byref(tn),
+                                             _ptr(hr) if want_hist else None, _ptr(ht) if want_hist else None,
+                                             ctypes.byref(ms)), "cg_mixed_dev")
+        return {"niters": nit.value, "true_normr": tn.value, "hist_rec": hr, "hist_true": ht, "status": st,
+                "ms": ms.value}
+
+    def spmv_mixed_dev(self, x: DeviceArray, y: DeviceArray):
+        _check(lib().hpccg_spmv_mixed_dev(self.h, x.ptr, y.ptr), "spmv_mixed_dev")
+
+    def time_spmv_mixed(self, x: DeviceArray, y: DeviceArray, reps=20):
+        ms = ctypes.c_double()
+        _check(lib().hpccg_time_spmv_mixed(self.h, x.ptr, y.ptr, reps, ctypes.byref(ms)), "time_spmv_mixed")
+        return ms.value
+
+    # ---- batched solves
+    def batch_setup(self, dims):
+        dims = np.ascontiguousarray(np.asarray(dims, dtype=np.int32).reshape(-1))
+        _check(lib().hpccg_batch_setup(self.h, len(dims) // 3, _ptr(dims)), "batch_setup")
+        return self.batch_info()
+
+    def batch_info(self):
+        cnt, E, nnz = ctypes.c_int32(), ctypes.c_int64(), ctypes.c_int64()
+        _check(lib().hpccg_batch_info(self.h, ctypes.byref(cnt), ctypes.byref(E), ctypes.byref(nnz), None))
+        off = np.zeros(cnt.value + 1, dtype=np.int64)
+        _check(lib().hpccg_batch_info(self.h, None, None, None, _ptr(off)))
+        return {"count": cnt.value, "rows": E.value, "nnz": nnz.value, "row_off": off}
+
+    def batch_spmv_dev(self, x: DeviceArray, y: DeviceArray):
+        _check(lib().hpccg_batch_spmv_dev(self.h, x.ptr, y.ptr), "batch_spmv_dev")
+
+    @staticmethod
+    def batch_variants():
+        n = ctypes.c_int32()
+        lib().hpccg_batch_variants(ctypes.byref(n), None, None)
+        rows, occ = np.zeros(n.value, np.int32), np.zeros(n.value, np.int32)
+        lib().hpccg_batch_variants(ctypes.byref(n), _ptr(rows), _ptr(occ))
+        return list(zip(rows.tolist(), occ.tolist()))
+
+    def batch_autotune(self, b: DeviceArray, x: DeviceArray, probe_iters=8):
+        nv = len(self.batch_variants())
+        ms = np.zeros(nv)
+        best = ctypes.c_int32()
+        _check(lib().hpccg_batch_autotune(self.h, b.ptr, x.ptr, probe_iters, ctypes.byref(best), _ptr(ms)),
+               "batch_autotune")
+        return best.value, ms
+
+    def batch_solve_dev(self, b: DeviceArray, x: DeviceArray, rel_tol=1e-8, max_iter=200, replace_every=25,
+                        variant=-1):
+        S = self.batch_info()["count"]
+        nit, rel, conv = np.zeros(S, np.int32), np.zeros(S), np.zeros(S, np.int32)
+        ms = ctypes.c_double()
+        st = _check(lib().hpccg_batch_solve_dev(self.h, b.ptr, x.ptr, rel_tol, max_iter, replace_every, variant,
+                                                _ptr(nit), _ptr(rel), _ptr(conv), ctypes.byref(ms)), "batch_solve")
+        return {"niters": nit, "relres": rel, "converged": conv, "ms": ms.value, "status": st}
 
     def spmv_dev(self, x: DeviceArray, y: DeviceArray):
         _check(lib().hpccg_spmv_dev(self.h, x.ptr, y.ptr), "spmv_dev")

@@ -115,6 +115,46 @@ int hpccg_wxbw_dev(hpccg_ctx *ctx, int32_t n, const double *x, double beta, doub
 int hpccg_cg_dev(hpccg_ctx *ctx, const double *b, double *x, int32_t max_iter, double tolerance,
                  int32_t *niters, double *normr, double *hist, double *elapsed_ms);
 
+/* ---- mixed precision (no reference counterpart: extends cg, numerics.F90:41-162) ----------
+ * FP16 matrix values (exact for generate_matrix's 27 / -1; other matrices are rejected unless
+ * every value is an FP16 number), int32 columns, FP32 r/p/Ap with FP64 arithmetic inside the
+ * kernels, FP64 dots for alpha/beta, FP64 x, and replacement of the recursive residual by the
+ * FP64 true residual b - A x every replace_every iterations (0 = never; the true residual is
+ * still formed whenever the recursive norm falls below the tolerance and at max_iter).
+ * Stops when the TRUE residual norm is below `tolerance` (absolute).  niters = iterations run;
+ * *true_normr = ||b - A x|| at exit; hist_rec[j] (j = 1..max_iter) the recursive ||r_j||;
+ * hist_true[j] (j = 0..max_iter) the true ||b - A x_j|| where formed, else -1. */
+int hpccg_cg_mixed(hpccg_ctx *ctx, const double *b, double *x, int32_t max_iter, double tolerance,
+                   int32_t replace_every, int32_t *niters, double *true_normr, double *hist_rec,
+                   double *hist_true);
+int hpccg_cg_mixed_dev(hpccg_ctx *ctx, const double *b, double *x, int32_t max_iter,
+                       double tolerance, int32_t replace_every, int32_t *niters, double *true_normr,
+                       double *hist_rec, double *hist_true, double *elapsed_ms);
+/* y = A x with FP16 values and FP32 vectors (FP64 accumulation); device arrays. */
+int hpccg_spmv_mixed_dev(hpccg_ctx *ctx, const float *x, float *y);
+int hpccg_time_spmv_mixed(hpccg_ctx *ctx, const float *x, float *y, int32_t reps, double *avg_ms);
+
+/* ---- batched solves (no reference counterpart: many independent generate_matrix grids,
+ * SparseMatrix.F90:68-185, each solved by the mixed-precision cg above) -------------------
+ * hpccg_batch_setup builds the block-diagonal CSR of `count` grids (dims = nx,ny,nz per
+ * system) on the device.  Vectors are concatenated in system order (row offsets from
+ * hpccg_batch_info, count+1 entries).  A solve is one launch of one captured graph. */
+int hpccg_batch_setup(hpccg_ctx *ctx, int32_t count, const int32_t *dims);
+int hpccg_batch_info(hpccg_ctx *ctx, int32_t *count, int64_t *total_rows, int64_t *total_nnz,
+                     int64_t *row_off);
+int hpccg_batch_spmv_dev(hpccg_ctx *ctx, const float *x, float *y);
+/* compiled launch variants: rows per workgroup tile and workgroups per CU */
+int hpccg_batch_variants(int32_t *count, int32_t *rows, int32_t *occ);
+/* autotuner: times every variant on this batch (probe_iters iterations each, x restored) and
+ * keeps the fastest for solves with variant = -1 */
+int hpccg_batch_autotune(hpccg_ctx *ctx, const double *b, double *x, int32_t probe_iters,
+                         int32_t *best, double *variant_ms);
+/* every system to ||b_s - A_s x_s|| <= rel_tol ||b_s|| (FP64 true residual) or max_iter;
+ * per-system niters / relres / converged (arrays of `count`) */
+int hpccg_batch_solve_dev(hpccg_ctx *ctx, const double *b, double *x, double rel_tol,
+                          int32_t max_iter, int32_t replace_every, int32_t variant,
+                          int32_t *niters, double *relres, int32_t *converged, double *elapsed_ms);
+
 /* ---- seeded synthetic inputs (no reference counterpart: the reference hard-codes x* = 1,
  * SparseMatrix.F90:170-172).  Host arrays; deterministic SplitMix64 streams. --------------- */
 /* x* = sum of 8 random low-frequency 3-D sines with random phases + U(-0.05,0.05) noise */

@@ -1,0 +1,98 @@
+"""GPU tests of the mixed-precision CG (FP16 values, FP32 vectors, FP64 dots / x, residual
+replacement).  It has no reference counterpart; it is judged by the FP64 oracle: the true
+residual ||b - A x|| / ||b|| of its answer (computed by the oracle in FP64) must meet the
+tolerance, and its error against x* must be as small as the FP64 reference solve's."""
+import math
+
+import numpy as np
+import pytest
+
+import hpccg_amd
+import oracle
+from conftest import load_golden
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = hpccg_amd.Context(0)
+    yield c
+    c.close()
+
+
+def test_spmv_mixed_is_fp64_sum_rounded_once(ctx):
+    """FP16 values x FP32 vector: products exact in FP64, row sums in the reference's order,
+    one rounding to FP32 -- bitwise equal to the oracle's FP64 spmv rounded to FP32."""
+    for dims in [(16, 8, 12), (33, 17, 5)]:
+        A = oracle.Problem(*dims)
+        ctx.upload(A.rowindex, A.colindex, A.values)
+        x32 = oracle.xstar_uniform(A.n, 77).astype(np.float32)
+        xd = hpccg_amd.DeviceArray.from_numpy(ctx, x32)
+        yd = hpccg_amd.DeviceArray(ctx, A.n, np.float32)
+        ctx.spmv_mixed_dev(xd, yd)
+        ref = oracle.spmv(A, x32.astype(np.float64)).astype(np.float32)
+        assert np.array_equal(yd.numpy(), ref)
+
+
+@pytest.mark.parametrize("grid,seed", [((32, 32, 32), 11), ((64, 64, 64), 12), ((40, 24, 16), 3)])
+def test_cg_mixed_reaches_fp64_true_residual(ctx, grid, seed):
+    A = oracle.Problem(*grid)
+    xs = oracle.xstar_sines(*grid, seed)
+    b = oracle.spmv(A, xs)
+    normb = math.sqrt(oracle.ddot1(b))
+    tol = 1e-10 * normb
+    ctx.upload(A.rowindex, A.colindex, A.values)
+    x, nit, tn, hr, ht, st = ctx.cg_mixed(b, max_iter=1000, tolerance=tol, replace_every=25)
+    assert st == 0
+    r = b - oracle.spmv(A, x)
+    true_rel = math.sqrt(oracle.ddot1(r)) / normb
+    assert true_rel <= 1e-10, true_rel
+    assert abs(tn / normb - true_rel) <= 1e-3 * true_rel  # the reported norm is the true one
+    _, nit64, _, _, _ = oracle.cg(A, b, max_iter=1000, tolerance=tol)
+    assert nit <= 1.3 * nit64 + 10, (nit, nit64)
+    xo, _, _, _, _ = oracle.cg(A, b, max_iter=1000, tolerance=tol)
+    e_mixed = float(np.max(np.abs(x - xs)))
+    e_fp64 = float(np.max(np.abs(xo - xs)))
+    assert e_mixed <= 10 * e_fp64 + 1e-12, (e_mixed, e_fp64)
+    # replacement points carry a true residual; Prepare's is ||b||
+    assert ht[0] == pytest.approx(normb, rel=1e-12)
+    assert all(ht[j] > 0 for j in range(25, nit, 25))
+
+
+def test_cg_mixed_device_api_matches_host_api(ctx):
+    A = oracle.Problem(24, 20, 16)
+    xs = oracle.xstar_sines(24, 20, 16, 4)
+    b = oracle.spmv(A, xs)
+    tol = 1e-10 * math.sqrt(oracle.ddot1(b))
+    ctx.upload(A.rowindex, A.colindex, A.values)
+    xh, nit, tn, hr, ht, _ = ctx.cg_mixed(b, max_iter=500, tolerance=tol)
+    bd = hpccg_amd.DeviceArray.from_numpy(ctx, b)
+    xd = hpccg_amd.DeviceArray.from_numpy(ctx, np.zeros(A.n))
+    res = ctx.cg_mixed_dev(bd, xd, max_iter=500, tolerance=tol)
+    assert res["niters"] == nit and np.array_equal(xd.numpy(), xh)  # deterministic
+    xd.upload(np.zeros(A.n))
+    res2 = ctx.cg_mixed_dev(bd, xd, max_iter=500, tolerance=tol)  # cached graph replay
+    assert res2["niters"] == nit and np.array_equal(xd.numpy(), xh)
+
+
+def test_cg_mixed_edges(ctx):
+    A = oracle.Problem(10, 9, 8)
+    ctx.upload(A.rowindex, A.colindex, A.values)
+    x, nit, tn, hr, ht, st = ctx.cg_mixed(A.b, max_iter=50, tolerance=1e9)
+    assert nit == 0 and np.all(x == 0) and tn == pytest.approx(math.sqrt(oracle.ddot1(A.b)))
+    x, nit, tn, hr, ht, st = ctx.cg_mixed(A.b, max_iter=7, tolerance=0.0, replace_every=0)
+    assert nit == 7 and ht[7] > 0  # max_iter forces a final true residual
+    r = A.b - oracle.spmv(A, x)
+    assert tn == pytest.approx(math.sqrt(oracle.ddot1(r)), rel=1e-6)
+    x, nit, tn, hr, ht, st = ctx.cg_mixed(A.b, max_iter=0)
+    assert nit == 0
+
+
+def test_cg_mixed_rejects_values_not_in_fp16(ctx):
+    A = oracle.Problem(5, 5, 5)
+    vals = A.values.copy()
+    vals[3] = 0.1
+    ctx.upload(A.rowindex, A.colindex, vals)
+    with pytest.raises(hpccg_amd.HpccgError):
+        ctx.cg_mixed(A.b, max_iter=5)
