@@ -408,7 +408,7 @@ __global__ void k_generate(int nx, int ny, int nz, int32_t *__restrict__ rowinde
 // ============================================================================================
 // host side
 // ============================================================================================
-static int spmv_grid(const hpccg_ctx *c, int ntiles) { return spmv_grid_for(c->info.num_cu, ntiles); }
+#define spmv_grid(c, k, nt) HPCCG_SPMV_GRID(k, (c)->info.num_cu, nt)
 static int vec_grid(const hpccg_ctx *c, int64_t n) { return vec_grid_for(c->info.num_cu, n); }
 
 static int ensure_workspace(hpccg_ctx *c, int64_t n)
@@ -473,12 +473,14 @@ static int matrix_required(const hpccg_ctx *c)
 }
 
 static int ntiles_of(const hpccg_ctx *c) { return (c->A.nrow + kTileRows - 1) / kTileRows; }
+// the tree and sequential SpMV kernels have the same shape; size both by the tree variant
+static int tree_or_seq_grid(const hpccg_ctx *c, int nt) { return spmv_grid(c, (k_cg_spmv<kTileRows, true>), nt); }
 
 int launch_spmv(hpccg_ctx *c, const double *x, double *y)
 {
     const int nt = ntiles_of(c);
     if (nt == 0) return 0;
-    hipLaunchKernelGGL(k_spmv<kTileRows>, dim3(spmv_grid(c, nt)), dim3(kTileRows), 0, c->stream, c->A.rowindex,
+    hipLaunchKernelGGL(k_spmv<kTileRows>, dim3(spmv_grid(c, k_spmv<kTileRows>, nt)), dim3(kTileRows), 0, c->stream, c->A.rowindex,
                        c->A.colindex, c->A.values, x, y, c->A.nrow, nt);
     HPCCG_HIP_TRY(hipGetLastError());
     return 0;
@@ -500,7 +502,7 @@ static int launch_dot(hpccg_ctx *c, int n, const double *x, const double *y, dou
 static int enqueue_iterations(hpccg_ctx *c, double *x, int count)
 {
     const int n = c->A.nrow, nt = ntiles_of(c);
-    const int Gs = spmv_grid(c, nt), Gv = vec_grid(c, n);
+    const int Gs = tree_or_seq_grid(c, nt), Gv = vec_grid(c, n);
     const bool tree = c->dot_mode == HPCCG_DOT_TREE;
     for (int it = 0; it < count; ++it) {
         hipLaunchKernelGGL(k_cg_update_p, dim3(Gv), dim3(kVecThreads), 0, c->stream, c->r, c->p, x, n, c->state);
@@ -574,7 +576,7 @@ static int cg_device(hpccg_ctx *c, const double *b, double *x, int max_iter, dou
 
     HPCCG_HIP_TRY(hipEventRecord(c->ev0, c->stream));
     // Prepare (numerics.F90:92-99): Ap = A x; r = b - Ap; rtrans = r.r
-    hipLaunchKernelGGL(k_cg_prep_spmv<kTileRows>, dim3(spmv_grid(c, nt)), dim3(kTileRows), 0, c->stream,
+    hipLaunchKernelGGL(k_cg_prep_spmv<kTileRows>, dim3(spmv_grid(c, k_cg_prep_spmv<kTileRows>, nt)), dim3(kTileRows), 0, c->stream,
                        c->A.rowindex, c->A.colindex, c->A.values, x, c->Ap, n, nt);
     if (tree) {
         hipLaunchKernelGGL(k_cg_prep_r<true>, dim3(vec_grid(c, n)), dim3(kVecThreads), 0, c->stream, b, c->Ap, c->r,
@@ -626,6 +628,54 @@ static int cg_device(hpccg_ctx *c, const double *b, double *x, int max_iter, dou
         for (int j = 0; j <= max_iter; ++j) hist[j] = (j <= nit) ? std::sqrt(rr[j]) : -1.0;
     }
     return s.breakdown ? HPCCG_WARN_BREAKDOWN : 0;
+}
+
+int cg_device_fp64(hpccg_ctx *c, const double *b, double *x, int max_iter, double tol, int *niters, double *normr,
+                   double *hist, double *elapsed_ms)
+{
+    return cg_device(c, b, x, max_iter, tol, niters, normr, hist, elapsed_ms);
+}
+
+int cg_trace(hpccg_ctx *c, const double *b, double *x, int max_iter, double tol, const std::function<int(int)> &hook)
+{
+    HPCCG_TRY(matrix_required(c));
+    const int n = c->A.nrow, nt = ntiles_of(c);
+    HPCCG_TRY(ensure_workspace(c, n));
+    HPCCG_TRY(ensure_hist(c, max_iter + 2));
+    const int Gs = spmv_grid(c, (k_cg_spmv<kTileRows, true>), nt), Gv = vec_grid(c, n);
+    CgState init{};
+    init.tol = tol;
+    init.stop_at = max_iter + 1;
+    init.max_iter = max_iter;
+    *c->state_h = init;
+    HPCCG_HIP_TRY(hipMemcpyAsync(c->state, c->state_h, sizeof(CgState), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_cg_prep_spmv<kTileRows>, dim3(spmv_grid(c, k_cg_prep_spmv<kTileRows>, nt)), dim3(kTileRows),
+                       0, c->stream, c->A.rowindex, c->A.colindex, c->A.values, x, c->Ap, n, nt);
+    hipLaunchKernelGGL(k_cg_prep_r<true>, dim3(Gv), dim3(kVecThreads), 0, c->stream, b, c->Ap, c->r, n, c->state,
+                       c->rr_hist, c->partials);
+    HPCCG_HIP_TRY(hipGetLastError());
+    // p_0 = r_0 and alpha = 0 make the hook's x + alpha p equal x_0
+    HPCCG_HIP_TRY(hipMemcpyAsync(c->p, c->r, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    HPCCG_HIP_TRY(hipStreamSynchronize(c->stream));
+    if (hook(0)) return 0;
+    for (int it = 1; it <= max_iter; ++it) {
+        HPCCG_HIP_TRY(hipMemcpyAsync(c->state_h, c->state, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
+        HPCCG_HIP_TRY(hipStreamSynchronize(c->stream));
+        if (c->state_h->k >= c->state_h->stop_at) break;
+        const int k = c->state_h->k;
+        hipLaunchKernelGGL(k_cg_update_p, dim3(Gv), dim3(kVecThreads), 0, c->stream, c->r, c->p, x, n, c->state);
+        hipLaunchKernelGGL((k_cg_spmv<kTileRows, true>), dim3(Gs), dim3(kTileRows), 0, c->stream, c->A.rowindex,
+                           c->A.colindex, c->A.values, c->p, c->Ap, n, nt, c->state, c->rr_hist, c->partials);
+        hipLaunchKernelGGL(k_cg_update_r<true>, dim3(Gv), dim3(kVecThreads), 0, c->stream, c->r, c->Ap, n, c->state,
+                           c->rr_hist, c->partials);
+        HPCCG_HIP_TRY(hipGetLastError());
+        HPCCG_HIP_TRY(hipStreamSynchronize(c->stream));
+        if (hook(k)) break;
+    }
+    hipLaunchKernelGGL(k_cg_finish_x, dim3(Gv), dim3(kVecThreads), 0, c->stream, c->p, x, n, c->state);
+    HPCCG_HIP_TRY(hipGetLastError());
+    HPCCG_HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
 }
 
 }  // namespace hpccg
@@ -1085,7 +1135,7 @@ int hpccg_time_spmv(hpccg_ctx *c, const double *x, double *y, int32_t reps, doub
     HPCCG_HIP_TRY(hipMemcpyAsync(c->state, c->state_h, sizeof(CgState), hipMemcpyHostToDevice, c->stream));
     const double one = 1.0;
     HPCCG_HIP_TRY(hipMemcpyAsync(c->rr_hist, &one, sizeof(double), hipMemcpyHostToDevice, c->stream));
-    const int n = c->A.nrow, nt = ntiles_of(c), G = spmv_grid(c, nt);
+    const int n = c->A.nrow, nt = ntiles_of(c), G = spmv_grid(c, (k_cg_spmv<kTileRows, true>), nt);
     auto launch = [&]() {
         hipLaunchKernelGGL((k_cg_spmv<kTileRows, true>), dim3(G), dim3(kTileRows), 0, c->stream, c->A.rowindex,
                            c->A.colindex, c->A.values, x, y, n, nt, c->state, c->rr_hist, c->partials);

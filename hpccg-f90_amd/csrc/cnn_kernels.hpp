@@ -1,0 +1,290 @@
+// Learned preconditioner: BF16 implicit-GEMM 3x3x3 convolution kernels on CDNA4 matrix cores.
+//
+// Network (bias-free, so positively homogeneous: net(c*u) = c*net(u) for c > 0):
+//   h1 = relu(conv(u; W1[16][27])), h2 = relu(conv(h1; W2[16][16][27])), z = conv(h2; W3[16][27])
+// conv(in; W)[co](v) = sum_{ci,t} W[co][ci][t] * in[ci](v + off(t)), off(t) = (t%3-1, t/3%3-1,
+// t/9-1) in (x, y, z), zero outside the grid.  Activations are channel-last BF16
+// ([batch][voxel][16], 32 bytes per voxel), accumulation is FP32 (MFMA).
+//
+// Implicit GEMM on v_mfma_f32_16x16x32_bf16 (D[16x16] += A[16x32] * B[32x16]):
+//   M = output channels (16), N = 16 voxels, K = 2 taps x 16 input channels per MFMA
+//   (14 MFMAs for the 27 taps, the 28th tap zero), or K = 27(+5) taps for a 1-channel input.
+//   A = weights, pre-packed per lane and held in registers for the whole kernel.
+//   B = activations from an LDS halo tile: each lane reads the 8 channels it needs as one
+//   16-byte LDS load.
+// Operand layout per lane l (wave64): A[m = l%16][k = 8*(l/16) + j], B[k = 8*(l/16) + j][n = l%16],
+// D[m = 4*(l/16) + i][n = l%16], j = 0..7, i = 0..3.
+//
+// Tiles: 8 x 8 x 4 output voxels per 256-thread workgroup (4 waves, one z-plane each, four
+// 16-voxel groups per wave), halo 10 x 10 x 6 staged in LDS.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+
+namespace hpccg {
+namespace cnn {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+constexpr int TX = 8, TY = 8, TZ = 4;
+constexpr int HX = TX + 2, HY = TY + 2, HZ = TZ + 2;
+constexpr int HV = HX * HY * HZ;  // 600 halo voxels
+constexpr int TV = TX * TY * TZ;  // 256 voxels per tile
+constexpr int NSTEP16 = 14;       // MFMA K-steps for 16 input channels x 27 taps
+constexpr int NW1 = 16 * 27, NW2 = 16 * 16 * 27, NW3 = 16 * 27;
+constexpr int NWALL = NW1 + NW2 + NW3;  // 7776 parameters
+
+// float -> bfloat16, round to nearest even (inf/nan pass through)
+__host__ __device__ inline uint16_t f2bf(float f)
+{
+    uint32_t u;
+    __builtin_memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__host__ __device__ inline float bf2f(uint16_t h)
+{
+    const uint32_t u = (uint32_t)h << 16;
+    float f;
+    __builtin_memcpy(&f, &u, 4);
+    return f;
+}
+
+__device__ inline int tap_dx(int t) { return t % 3 - 1; }
+__device__ inline int tap_dy(int t) { return (t / 3) % 3 - 1; }
+__device__ inline int tap_dz(int t) { return t / 9 - 1; }
+
+__device__ inline f32x4 mfma16(bf16x8 a, bf16x8 b, f32x4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+struct Grid3 {
+    int nx, ny, nz;
+    int tx, ty, tz;  // tiles per axis
+    __host__ __device__ int64_t nvox() const { return (int64_t)nx * ny * nz; }
+    __host__ __device__ int ntiles() const { return tx * ty * tz; }
+};
+
+inline Grid3 make_grid(int nx, int ny, int nz)
+{
+    return Grid3{nx, ny, nz, (nx + TX - 1) / TX, (ny + TY - 1) / TY, (nz + TZ - 1) / TZ};
+}
+
+// Input of a convolution: BF16 channel-last activations, or (layer 1) an FP32 field multiplied
+// by a per-sample scale and rounded to BF16 while it is staged.
+enum InKind { IN_BF16_16 = 0, IN_BF16_1 = 1, IN_F32_1 = 2 };
+// Epilogue: ReLU -> BF16 [v][16]; FP32 channel 0 times a per-sample scale; masked BF16 (the
+// backward pass through a ReLU whose forward output is `mask`).
+enum OutKind { OUT_RELU_BF16 = 0, OUT_F32_CH0 = 1, OUT_MASKED_BF16 = 2 };
+
+// Stage the 10x10x6 halo of tile (ox,oy,oz) of sample b into LDS (channel-last, 16 or 1 ch).
+template <int IK>
+__device__ inline void stage_halo(const void *in, float scale, uint16_t *sm, const Grid3 &g, int b, int ox, int oy,
+                                  int oz)
+{
+    const int64_t N = g.nvox();
+    if (IK == IN_BF16_16) {
+        const uint16_t *src = (const uint16_t *)in;
+        for (int i = threadIdx.x; i < HV * 2; i += blockDim.x) {
+            const int hv = i >> 1, ch = i & 1;
+            const int hx = hv % HX, hy = (hv / HX) % HY, hz = hv / (HX * HY);
+            const int gx = ox - 1 + hx, gy = oy - 1 + hy, gz = oz - 1 + hz;
+            u32x4 v = {0u, 0u, 0u, 0u};
+            if (gx >= 0 && gx < g.nx && gy >= 0 && gy < g.ny && gz >= 0 && gz < g.nz) {
+                const int64_t gv = ((int64_t)gz * g.ny + gy) * g.nx + gx;
+                v = *(const u32x4 *)(src + ((int64_t)b * N + gv) * 16 + ch * 8);
+            }
+            *(u32x4 *)(sm + hv * 16 + ch * 8) = v;
+        }
+    } else {
+        for (int hv = threadIdx.x; hv < HV; hv += blockDim.x) {
+            const int hx = hv % HX, hy = (hv / HX) % HY, hz = hv / (HX * HY);
+            const int gx = ox - 1 + hx, gy = oy - 1 + hy, gz = oz - 1 + hz;
+            uint16_t v = 0;
+            if (gx >= 0 && gx < g.nx && gy >= 0 && gy < g.ny && gz >= 0 && gz < g.nz) {
+                const int64_t gv = (int64_t)b * N + ((int64_t)gz * g.ny + gy) * g.nx + gx;
+                v = IK == IN_F32_1 ? f2bf(((const float *)in)[gv] * scale) : ((const uint16_t *)in)[gv];
+            }
+            sm[hv] = v;
+        }
+    }
+}
+
+// One 3x3x3 convolution layer with 16 output channels (layer 3 packs its single output channel
+// as channel 0 of 16).  Grid: one workgroup per (tile, sample).
+template <int IK, int OK>
+__global__ __launch_bounds__(256) void k_conv(const void *__restrict__ in, const float *__restrict__ in_scale,
+                                              const bf16x8 *__restrict__ wpack, void *__restrict__ out,
+                                              const uint16_t *__restrict__ mask, const float *__restrict__ out_scale,
+                                              Grid3 g, const int *__restrict__ skip)
+{
+    if (skip && *skip) return;
+    constexpr int CIN = IK == IN_BF16_16 ? 16 : 1;
+    constexpr int NSTEP = CIN == 16 ? NSTEP16 : 1;
+    __shared__ __attribute__((aligned(16))) uint16_t sm[HV * CIN];
+    const int nt = g.ntiles();
+    const int b = blockIdx.x / nt, tile = blockIdx.x % nt;
+    const int ox = (tile % g.tx) * TX, oy = ((tile / g.tx) % g.ty) * TY, oz = (tile / (g.tx * g.ty)) * TZ;
+    const float scale = (IK == IN_F32_1 && in_scale) ? in_scale[b] : 1.0f;
+    stage_halo<IK>(in, scale, sm, g, b, ox, oy, oz);
+
+    const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+    const int c = lane / 16, n = lane % 16;
+    bf16x8 A[NSTEP];
+#pragma unroll
+    for (int s = 0; s < NSTEP; ++s) A[s] = wpack[s * 64 + lane];
+    __syncthreads();
+
+    const int64_t N = g.nvox();
+    const int z = w;
+#pragma unroll 1
+    for (int q = 0; q < 4; ++q) {
+        const int x = n % 8, y = 2 * q + n / 8;
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        if (CIN == 16) {
+#pragma unroll
+            for (int s = 0; s < NSTEP; ++s) {
+                const int t = 2 * s + c / 2;
+                u32x4 raw = {0u, 0u, 0u, 0u};
+                if (t < 27) {
+                    const int hv = (z + tap_dz(t) + 1) * (HY * HX) + (y + tap_dy(t) + 1) * HX + (x + tap_dx(t) + 1);
+                    raw = *(const u32x4 *)(sm + hv * 16 + (c & 1) * 8);
+                }
+                acc = mfma16(A[s], __builtin_bit_cast(bf16x8, raw), acc);
+            }
+        } else {
+            uint16_t v[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int t = 8 * c + j;
+                v[j] = 0;
+                if (t < 27) {
+                    const int hv = (z + tap_dz(t) + 1) * (HY * HX) + (y + tap_dy(t) + 1) * HX + (x + tap_dx(t) + 1);
+                    v[j] = sm[hv];
+                }
+            }
+            u32x4 raw;
+            raw[0] = v[0] | ((uint32_t)v[1] << 16);
+            raw[1] = v[2] | ((uint32_t)v[3] << 16);
+            raw[2] = v[4] | ((uint32_t)v[5] << 16);
+            raw[3] = v[6] | ((uint32_t)v[7] << 16);
+            acc = mfma16(A[0], __builtin_bit_cast(bf16x8, raw), acc);
+        }
+        // epilogue: lane holds channels 4c..4c+3 of voxel (x, y, z) of this tile
+        const int gx = ox + x, gy = oy + y, gz = oz + z;
+        if (gx < g.nx && gy < g.ny && gz < g.nz) {
+            const int64_t gv = (int64_t)b * N + ((int64_t)gz * g.ny + gy) * g.nx + gx;
+            if (OK == OUT_F32_CH0) {
+                if (c == 0) ((float *)out)[gv] = acc[0] * (out_scale ? out_scale[b] : 1.0f);
+            } else {
+                uint16_t o[4];
+                if (OK == OUT_RELU_BF16) {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) o[i] = f2bf(acc[i] > 0.f ? acc[i] : 0.f);
+                } else {
+                    const u32x2 mm = *(const u32x2 *)(mask + gv * 16 + 4 * c);
+                    const uint16_t m4[4] = {(uint16_t)(mm[0] & 0xffff), (uint16_t)(mm[0] >> 16),
+                                            (uint16_t)(mm[1] & 0xffff), (uint16_t)(mm[1] >> 16)};
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) o[i] = bf2f(m4[i]) > 0.f ? f2bf(acc[i]) : (uint16_t)0;
+                }
+                u32x2 pk;
+                pk[0] = o[0] | ((uint32_t)o[1] << 16);
+                pk[1] = o[2] | ((uint32_t)o[3] << 16);
+                *(u32x2 *)((uint16_t *)out + gv * 16 + 4 * c) = pk;
+            }
+        }
+    }
+}
+
+// Weight gradient dW[co][ci][t] = sum_v dout[v][co] * in[v + off(t)][ci] for CO_IN output
+// channels of dout (16, or 1 for dL/dz) and CI_IN input channels (16, or 1 for u): an MFMA
+// GEMM with M = co, N = ci, K = voxels.  Each workgroup walks a contiguous range of
+// (tile, sample) items and writes its FP32 partial [16][16][27] (padded) to partials[wg];
+// k_wgrad_reduce sums the partials in a fixed order.
+template <int CO_IN, int CI_IN>
+__global__ __launch_bounds__(256) void k_wgrad(const uint16_t *__restrict__ dout, const uint16_t *__restrict__ in,
+                                               float *__restrict__ partials, Grid3 g, int batch)
+{
+    __shared__ __attribute__((aligned(16))) uint16_t doutT[16 * TV];           // [co][voxel]
+    __shared__ __attribute__((aligned(16))) uint16_t inT[3 * 16 * HZ * HY * 8];  // [shift][ci][hz][hy][x]
+    const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+    const int c = lane / 16, n = lane % 16;
+    const int items = g.ntiles() * batch;
+    const int i0 = (int)(((int64_t)blockIdx.x * items) / gridDim.x);
+    const int i1 = (int)(((int64_t)(blockIdx.x + 1) * items) / gridDim.x);
+    const int64_t N = g.nvox();
+    f32x4 acc[7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) acc[k] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int it = i0; it < i1; ++it) {
+        const int b = it / g.ntiles(), tile = it % g.ntiles();
+        const int ox = (tile % g.tx) * TX, oy = ((tile / g.tx) % g.ty) * TY, oz = (tile / (g.tx * g.ty)) * TZ;
+        __syncthreads();
+        // dout tile, transposed to [co][voxel]; voxels outside the grid contribute zero
+        for (int i = threadIdx.x; i < TV * 16; i += blockDim.x) {
+            const int v = i / 16, co = i % 16;
+            const int x = v % TX, y = (v / TX) % TY, z = v / (TX * TY);
+            const int gx = ox + x, gy = oy + y, gz = oz + z;
+            uint16_t d = 0;
+            if (co < CO_IN && gx < g.nx && gy < g.ny && gz < g.nz) {
+                const int64_t gv = (int64_t)b * N + ((int64_t)gz * g.ny + gy) * g.nx + gx;
+                d = dout[gv * CO_IN + co];
+            }
+            doutT[co * TV + v] = d;
+        }
+        // input halo, transposed to [ci][hz][hy][hx] and stored three times, shifted by
+        // 0/1/2 in x, so that every 8-voxel x-run of a tap is one aligned 16-byte load
+        for (int i = threadIdx.x; i < 16 * HZ * HY * 8; i += blockDim.x) {
+            const int x = i % 8, hy = (i / 8) % HY, hz = (i / (8 * HY)) % HZ, ci = i / (8 * HY * HZ);
+            const int gy = oy - 1 + hy, gz = oz - 1 + hz;
+#pragma unroll
+            for (int s = 0; s < 3; ++s) {
+                const int gx = ox - 1 + x + s;
+                uint16_t v = 0;
+                if (ci < CI_IN && gx >= 0 && gx < g.nx && gy >= 0 && gy < g.ny && gz >= 0 && gz < g.nz) {
+                    const int64_t gv = (int64_t)b * N + ((int64_t)gz * g.ny + gy) * g.nx + gx;
+                    v = in[gv * CI_IN + ci];
+                }
+                inT[((s * 16 + ci) * HZ + hz) * (HY * 8) + hy * 8 + x] = v;
+            }
+        }
+        __syncthreads();
+#pragma unroll 1
+        for (int ks = 0; ks < TV / 32; ++ks) {
+            // K = 32 voxels = 4 x-rows of 8; this lane's 8 voxels: row r = 4*ks + c
+            const int row = 4 * ks + c;
+            const int y = row % TY, z = row / TY;
+            const bf16x8 a = __builtin_bit_cast(bf16x8, *(const u32x4 *)(doutT + n * TV + row * 8));
+#pragma unroll
+            for (int k = 0; k < 7; ++k) {
+                const int t = w + 4 * k;
+                if (t < 27) {
+                    const int s = tap_dx(t) + 1, hz = z + tap_dz(t) + 1, hy = y + tap_dy(t) + 1;
+                    const bf16x8 bb = __builtin_bit_cast(
+                        bf16x8, *(const u32x4 *)(inT + ((s * 16 + n) * HZ + hz) * (HY * 8) + hy * 8));
+                    acc[k] = mfma16(a, bb, acc[k]);
+                }
+            }
+        }
+    }
+    // partial[wg][(co*16 + ci)*27 + t]: lane holds co = 4c + i, ci = n
+    float *P = partials + (int64_t)blockIdx.x * (16 * 16 * 27);
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+        const int t = w + 4 * k;
+        if (t < 27)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) P[((4 * c + i) * 16 + n) * 27 + t] = acc[k][i];
+    }
+}
+
+}  // namespace cnn
+}  // namespace hpccg

@@ -3,6 +3,7 @@
 
 #include <hip/hip_fp16.h>
 
+#include <functional>
 #include <map>
 #include <tuple>
 
@@ -90,5 +91,15 @@ struct hpccg_ctx {
 namespace hpccg {
 void free_mixed(hpccg_ctx *c);
 void free_batched(hpccg_ctx *c);
+void free_learned(hpccg_ctx *c);
+int ensure_values16(hpccg_ctx *c);
+int cg_device_fp64(hpccg_ctx *c, const double *b, double *x, int max_iter, double tol, int *niters,
+                   double *normr, double *hist, double *elapsed_ms);
+// FP64 cg (tree dots) on device arrays without graphs, calling hook(k) after the residual
+// update of every iteration k that ran (k = 0 after Prepare).  During hook(k): c->r holds r_k,
+// c->p holds p_k, c->state->alpha holds alpha_k, and x holds x_{k-1} (x_k = x + alpha_k p_k;
+// the x update is applied at the top of the next iteration).  A nonzero hook return stops.
+int cg_trace(hpccg_ctx *c, const double *b, double *x, int max_iter, double tol,
+             const std::function<int(int)> &hook);
 int launch_spmv(hpccg_ctx *c, const double *x, double *y);
 }  // namespace hpccg

@@ -5,6 +5,7 @@ extern "C" void hpccg_free_ext(hpccg_ctx *c)
 {
     hpccg::free_mixed(c);
     hpccg::free_batched(c);
+    hpccg::free_learned(c);
     for (void *&p : c->ext) {
         if (p) hipFree(p);
         p = nullptr;

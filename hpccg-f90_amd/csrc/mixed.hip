@@ -196,7 +196,7 @@ __global__ __launch_bounds__(kVecThreads) void km_update_xr(double *__restrict__
 }
 
 // ---- host ------------------------------------------------------------------------------------
-static int ensure_values16(hpccg_ctx *c)
+int ensure_values16(hpccg_ctx *c)
 {
     if (c->A.values16) return 0;
     const int64_t nnz = c->A.nnz;
@@ -274,7 +274,7 @@ static int mixed_enqueue(hpccg_ctx *c, const double *b, double *x, int count)
 {
     MixedWs &m = c->mix;
     const int n = c->A.nrow, nt = (n + kTileRows - 1) / kTileRows;
-    const int Gs = spmv_grid_for(c->info.num_cu, nt), Gv = vec_grid_for(c->info.num_cu, n);
+    const int Gs = HPCCG_SPMV_GRID(km_spmv<kTileRows>, c->info.num_cu, nt), Gv = vec_grid_for(c->info.num_cu, n);
     MixState *st = (MixState *)m.state;
     double *hrec = m.hist, *htrue = m.hist + m.hist_cap;
     for (int it = 0; it < count; ++it) {
@@ -341,8 +341,8 @@ static int cg_mixed_device(hpccg_ctx *c, const double *b, double *x, int max_ite
     if (max_iter > 0) HPCCG_TRY(mixed_graph(c, b, x, chunk, &ge));
 
     HPCCG_HIP_TRY(hipEventRecord(c->ev0, c->stream));
-    hipLaunchKernelGGL((km_true_residual<kTileRows, true>), dim3(spmv_grid_for(c->info.num_cu, nt)), dim3(kTileRows), 0,
-                       c->stream, c->A.rowindex, c->A.colindex, c->A.values16, x, b, m.r, n, nt, (MixState *)m.state,
+    hipLaunchKernelGGL((km_true_residual<kTileRows, true>), dim3(HPCCG_SPMV_GRID((km_true_residual<kTileRows, true>), c->info.num_cu, nt)),
+                       dim3(kTileRows), 0, c->stream, c->A.rowindex, c->A.colindex, c->A.values16, x, b, m.r, n, nt, (MixState *)m.state,
                        m.hist + m.hist_cap, c->partials);
     HPCCG_HIP_TRY(hipGetLastError());
     int launched = 0;
@@ -455,7 +455,7 @@ int hpccg_spmv_mixed_dev(hpccg_ctx *c, const float *x, float *y)
     std::memcpy(c->mix.state_h, &init, sizeof(init));
     HPCCG_HIP_TRY(hipMemcpyAsync(c->mix.state, c->mix.state_h, sizeof(MixState), hipMemcpyHostToDevice, c->stream));
     const int n = c->A.nrow, nt = (n + kTileRows - 1) / kTileRows;
-    hipLaunchKernelGGL(km_spmv<kTileRows>, dim3(spmv_grid_for(c->info.num_cu, nt)), dim3(kTileRows), 0, c->stream,
+    hipLaunchKernelGGL(km_spmv<kTileRows>, dim3(HPCCG_SPMV_GRID(km_spmv<kTileRows>, c->info.num_cu, nt)), dim3(kTileRows), 0, c->stream,
                        c->A.rowindex, c->A.colindex, c->A.values16, x, y, n, nt, (MixState *)c->mix.state,
                        c->partials);
     HPCCG_HIP_TRY(hipGetLastError());
@@ -468,7 +468,7 @@ int hpccg_time_spmv_mixed(hpccg_ctx *c, const float *x, float *y, int32_t reps, 
     HPCCG_CHECK_ARG(c && x && y && reps > 0 && avg_ms, "bad argument");
     HPCCG_TRY(hpccg_spmv_mixed_dev(c, x, y));  // sets up state and warms
     const int n = c->A.nrow, nt = (n + kTileRows - 1) / kTileRows;
-    const int G = spmv_grid_for(c->info.num_cu, nt);
+    const int G = HPCCG_SPMV_GRID(km_spmv<kTileRows>, c->info.num_cu, nt);
     HPCCG_HIP_TRY(hipEventRecord(c->ev0, c->stream));
     for (int i = 0; i < reps; ++i)
         hipLaunchKernelGGL(km_spmv<kTileRows>, dim3(G), dim3(kTileRows), 0, c->stream, c->A.rowindex, c->A.colindex,

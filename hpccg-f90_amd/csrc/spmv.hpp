@@ -1,11 +1,13 @@
 // CSR SpMV building blocks shared by the FP64, mixed-precision and batched paths.
 #pragma once
 
+#include <hip/hip_fp16.h>
+
 #include "common.hpp"
 
 namespace hpccg {
 
-constexpr int kTileRows = 128;  // rows per SpMV tile == threads per SpMV workgroup
+constexpr int kTileRows = 64;  // rows per SpMV tile == threads per SpMV workgroup (one wave)
 constexpr int kVecThreads = 256;
 constexpr int kMaxGrid = 16384;
 
@@ -31,7 +33,7 @@ __device__ __forceinline__ __half ld_stream(const __half *p)
 // order.  VT = stored value type (double, or __half holding {27,-1} exactly), XT = vector
 // type.  With VT = double and XT = double the result is bitwise the reference's spmv.
 // Returns the row sum (FP64) of this thread's row, or 0 past the last row.
-template <int ROWS, typename VT, typename XT>
+template <int ROWS, typename VT, typename XT, int UNR = 16>
 __device__ __forceinline__ double spmv_tile_rowsum(const int32_t *__restrict__ rowptr,
                                                    const int32_t *__restrict__ col,
                                                    const VT *__restrict__ val,
@@ -53,11 +55,11 @@ __device__ __forceinline__ double spmv_tile_rowsum(const int32_t *__restrict__ r
     }
     double sum = 0.0;
     if (end - base <= ROWS * 27) {
-        for (int k = base + t; k < end; k += 4 * ROWS) {
-            VT v[4];
-            int c[4];
+        for (int k = base + t; k < end; k += UNR * ROWS) {
+            VT v[UNR];
+            int c[UNR];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < UNR; ++u) {
                 const int kk = k + u * ROWS;
                 if (kk < end) {
                     v[u] = ld_stream(val + kk);
@@ -65,7 +67,7 @@ __device__ __forceinline__ double spmv_tile_rowsum(const int32_t *__restrict__ r
                 }
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < UNR; ++u) {
                 const int kk = k + u * ROWS;
                 if (kk < end) lds[kk - base] = to_f64(v[u]) * to_f64(x[c[u] - 1]);
             }
@@ -100,14 +102,25 @@ __device__ __forceinline__ void elem_range(int64_t n, int64_t &i0, int64_t &i1, 
     i1 = (n * (lb + 1)) / G;
 }
 
-// Launch shapes (host): SpMV workgroups (multiple of 8 for the XCD renumbering) and
-// elementwise workgroups.
-inline int spmv_grid_for(int num_cu, int ntiles)
+// Launch shapes (host).  SpMV kernels run one resident wave of workgroups (CUs x blocks that
+// fit per CU, from the occupancy calculator), each walking a contiguous tile range, so there
+// is no partial second wave; the count is a multiple of 8 for the XCD renumbering.
+inline int spmv_grid_for(int num_cu, int ntiles, int blocks_per_cu)
 {
-    int G = ntiles < num_cu * 8 ? ntiles : num_cu * 8;
+    int G = ntiles < num_cu * blocks_per_cu ? ntiles : num_cu * blocks_per_cu;
     G = (G / kNumXcd) * kNumXcd;
     return G < kNumXcd ? kNumXcd : G;
 }
+template <typename K>
+inline int resident_blocks(K kernel, int block)
+{
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, block, 0) != hipSuccess || occ < 1) occ = 1;
+    return occ;
+}
+// grid for kernel `k` launched with kTileRows threads over ntiles tiles
+#define HPCCG_SPMV_GRID(k, num_cu, ntiles) \
+    ::hpccg::spmv_grid_for((num_cu), (ntiles), ::hpccg::resident_blocks((k), kTileRows))
 inline int vec_grid_for(int num_cu, int64_t n)
 {
     int64_t G = (n + kVecThreads - 1) / kVecThreads;

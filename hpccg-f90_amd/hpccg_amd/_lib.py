@@ -63,6 +63,18 @@ _SIGS = {
     "hpccg_batch_variants": (ctypes.c_int, [PI32, P, P]),
     "hpccg_batch_autotune": (ctypes.c_int, [P, P, P, I32, PI32, P]),
     "hpccg_batch_solve_dev": (ctypes.c_int, [P, P, P, D, I32, I32, I32, P, P, P, PD]),
+    "hpccg_cnn_num_weights": (ctypes.c_int, []),
+    "hpccg_cnn_init": (ctypes.c_int, [P, ctypes.c_uint64]),
+    "hpccg_cnn_set_weights": (ctypes.c_int, [P, P]),
+    "hpccg_cnn_get_weights": (ctypes.c_int, [P, P]),
+    "hpccg_cnn_forward": (ctypes.c_int, [P, I32, I32, I32, I32, P, P, P, P]),
+    "hpccg_cnn_backward": (ctypes.c_int, [P, I32, I32, I32, I32, P, P, P]),
+    "hpccg_cnn_make_dataset": (ctypes.c_int, [P, I32, I32, I32, ctypes.c_uint64]),
+    "hpccg_cnn_dataset_pair": (ctypes.c_int, [P, I32, P, P, P]),
+    "hpccg_cnn_train": (ctypes.c_int, [P, I32, I32, ctypes.c_float, ctypes.c_uint64, P, PD]),
+    "hpccg_cnn_apply_dev": (ctypes.c_int, [P, I32, I32, I32, P, ctypes.c_float, P]),
+    "hpccg_fcg_learned_dev": (ctypes.c_int, [P, I32, I32, I32, P, P, I32, D, I32, I32, D, PI32, PI32, PI32, PD,
+                                             P, PD]),
     "hpccg_xstar_sines": (ctypes.c_int, [I32, I32, I32, ctypes.c_uint64, P]),
     "hpccg_xstar_uniform": (ctypes.c_int, [I64, ctypes.c_uint64, P]),
     "hpccg_batch_dims": (ctypes.c_int, [I32, I32, I32, ctypes.c_uint64, P]),
@@ -371,6 +383,75 @@ class Context:
         st = _check(lib().hpccg_batch_solve_dev(self.h, b.ptr, x.ptr, rel_tol, max_iter, replace_every, variant,
                                                 _ptr(nit), _ptr(rel), _ptr(conv), ctypes.byref(ms)), "batch_solve")
         return {"niters": nit, "relres": rel, "converged": conv, "ms": ms.value, "status": st}
+
+    # ---- learned preconditioner
+    @staticmethod
+    def cnn_num_weights():
+        return lib().hpccg_cnn_num_weights()
+
+    def cnn_init(self, seed=1):
+        _check(lib().hpccg_cnn_init(self.h, seed), "cnn_init")
+
+    def cnn_set_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        assert w.size == self.cnn_num_weights()
+        _check(lib().hpccg_cnn_set_weights(self.h, _ptr(w)), "cnn_set_weights")
+
+    def cnn_get_weights(self):
+        w = np.zeros(self.cnn_num_weights(), np.float32)
+        _check(lib().hpccg_cnn_get_weights(self.h, _ptr(w)), "cnn_get_weights")
+        return w
+
+    def cnn_forward(self, u, dims, batch=1, want_act=False):
+        nx, ny, nz = dims
+        u = np.ascontiguousarray(u, dtype=np.float32)
+        V = nx * ny * nz * batch
+        z = np.zeros(V, np.float32)
+        h1 = np.zeros(V * 16, np.float32) if want_act else None
+        h2 = np.zeros(V * 16, np.float32) if want_act else None
+        _check(lib().hpccg_cnn_forward(self.h, nx, ny, nz, batch, _ptr(u), _ptr(h1) if want_act else None,
+                                       _ptr(h2) if want_act else None, _ptr(z)), "cnn_forward")
+        return (z, h1, h2) if want_act else z
+
+    def cnn_backward(self, u, dz, dims, batch=1):
+        nx, ny, nz = dims
+        u = np.ascontiguousarray(u, dtype=np.float32)
+        dz = np.ascontiguousarray(dz, dtype=np.float32)
+        dW = np.zeros(self.cnn_num_weights(), np.float32)
+        _check(lib().hpccg_cnn_backward(self.h, nx, ny, nz, batch, _ptr(u), _ptr(dz), _ptr(dW)), "cnn_backward")
+        return dW
+
+    def cnn_make_dataset(self, n=64, npairs=20000, crop=24, seed=7):
+        _check(lib().hpccg_cnn_make_dataset(self.h, n, npairs, crop, seed), "cnn_make_dataset")
+
+    def cnn_dataset_pair(self, i, crop):
+        r = np.zeros(crop ** 3, np.float32)
+        e = np.zeros(crop ** 3, np.float32)
+        m = np.zeros(6, np.int32)
+        _check(lib().hpccg_cnn_dataset_pair(self.h, i, _ptr(r), _ptr(e), _ptr(m)), "cnn_dataset_pair")
+        return r, e, m
+
+    def cnn_train(self, steps, batch=32, lr=2e-3, seed=11):
+        loss = np.zeros(steps, np.float32)
+        ms = ctypes.c_double()
+        _check(lib().hpccg_cnn_train(self.h, steps, batch, lr, seed, _ptr(loss), ctypes.byref(ms)), "cnn_train")
+        return loss, ms.value
+
+    def cnn_apply_dev(self, dims, r: DeviceArray, inv_scale, z: DeviceArray):
+        _check(lib().hpccg_cnn_apply_dev(self.h, *dims, r.ptr, inv_scale, z.ptr), "cnn_apply_dev")
+
+    def fcg_learned_dev(self, dims, b: DeviceArray, x: DeviceArray, max_iter=1000, tolerance=0.0, replace_every=25,
+                        window=10, min_factor=2.0, want_hist=False):
+        fi, ci, fb = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        tn, ms = ctypes.c_double(), ctypes.c_double()
+        hist = np.zeros(max_iter + 1) if want_hist else None
+        st = _check(lib().hpccg_fcg_learned_dev(self.h, *dims, b.ptr, x.ptr, max_iter, tolerance, replace_every,
+                                                window, min_factor, ctypes.byref(fi), ctypes.byref(ci),
+                                                ctypes.byref(fb), ctypes.byref(tn),
+                                                _ptr(hist) if want_hist else None, ctypes.byref(ms)),
+                    "fcg_learned_dev")
+        return {"fcg_iters": fi.value, "cg_iters": ci.value, "fell_back": bool(fb.value), "true_normr": tn.value,
+                "ms": ms.value, "status": st, "hist": hist}
 
     def spmv_dev(self, x: DeviceArray, y: DeviceArray):
         _check(lib().hpccg_spmv_dev(self.h, x.ptr, y.ptr), "spmv_dev")

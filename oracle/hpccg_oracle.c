@@ -276,3 +276,117 @@ void oracle_batch_dims(int count, int lo, int hi, uint64_t seed, int32_t *dims)
     const uint64_t span = (uint64_t)(hi - lo + 1);
     for (int j = 0; j < 3 * count; ++j) dims[j] = lo + (int32_t)((next64(&s) >> 33) % span);
 }
+
+/* ------------------------------------------------------------------------------------------ */
+/* Learned preconditioner CNN (no reference counterpart).                                     */
+#include <string.h>
+
+float oracle_bf16(float v)
+{
+    uint32_t u;
+    memcpy(&u, &v, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u) return v; /* inf / nan unchanged */
+    u += 0x7fffu + ((u >> 16) & 1u);
+    u &= 0xffff0000u;
+    float r;
+    memcpy(&r, &u, 4);
+    return r;
+}
+
+static float q_(float v, int quant) { return quant ? oracle_bf16(v) : v; }
+
+/* out[b][v][co] = sum_{ci,t} W[co][ci][t] in[b][v+off(t)][ci], channel-last, zero padding;
+ * flip != 0 applies the transposed-flipped kernel W[ci][co][26-t] (backward data). */
+static void conv3(int nx, int ny, int nz, int batch, int cin, int cout, const float *W, int flip,
+                  const float *in, double *out)
+{
+    const int64_t N = (int64_t)nx * ny * nz;
+    for (int b = 0; b < batch; ++b)
+        for (int z = 0; z < nz; ++z)
+            for (int y = 0; y < ny; ++y)
+                for (int x = 0; x < nx; ++x) {
+                    const int64_t v = ((int64_t)z * ny + y) * nx + x;
+                    double acc[16] = {0};
+                    for (int t = 0; t < 27; ++t) {
+                        const int xx = x + t % 3 - 1, yy = y + (t / 3) % 3 - 1, zz = z + t / 9 - 1;
+                        if (xx < 0 || xx >= nx || yy < 0 || yy >= ny || zz < 0 || zz >= nz) continue;
+                        const float *iv = in + ((int64_t)b * N + ((int64_t)zz * ny + yy) * nx + xx) * cin;
+                        for (int co = 0; co < cout; ++co)
+                            for (int ci = 0; ci < cin; ++ci) {
+                                const float w = flip ? W[((int64_t)ci * cout + co) * 27 + (26 - t)]
+                                                     : W[((int64_t)co * cin + ci) * 27 + t];
+                                acc[co] += (double)w * (double)iv[ci];
+                            }
+                    }
+                    for (int co = 0; co < cout; ++co) out[((int64_t)b * N + v) * cout + co] = acc[co];
+                }
+}
+
+/* dW[co][ci][t] += sum_v dout[v][co] * in[v+off(t)][ci] */
+static void conv3_wgrad(int nx, int ny, int nz, int batch, int cin, int cout, const float *dout,
+                        const float *in, double *dW)
+{
+    const int64_t N = (int64_t)nx * ny * nz;
+    for (int b = 0; b < batch; ++b)
+        for (int z = 0; z < nz; ++z)
+            for (int y = 0; y < ny; ++y)
+                for (int x = 0; x < nx; ++x) {
+                    const int64_t v = ((int64_t)z * ny + y) * nx + x;
+                    const float *dv = dout + ((int64_t)b * N + v) * cout;
+                    for (int t = 0; t < 27; ++t) {
+                        const int xx = x + t % 3 - 1, yy = y + (t / 3) % 3 - 1, zz = z + t / 9 - 1;
+                        if (xx < 0 || xx >= nx || yy < 0 || yy >= ny || zz < 0 || zz >= nz) continue;
+                        const float *iv = in + ((int64_t)b * N + ((int64_t)zz * ny + yy) * nx + xx) * cin;
+                        for (int co = 0; co < cout; ++co)
+                            for (int ci = 0; ci < cin; ++ci)
+                                dW[((int64_t)co * cin + ci) * 27 + t] += (double)dv[co] * (double)iv[ci];
+                    }
+                }
+}
+
+static float *qcopy(const float *a, int64_t n, int quant)
+{
+    float *o = (float *)malloc(sizeof(float) * (size_t)n);
+    for (int64_t i = 0; i < n; ++i) o[i] = q_(a[i], quant);
+    return o;
+}
+
+void oracle_cnn_forward(int nx, int ny, int nz, int batch, const float *W1, const float *W2,
+                        const float *W3, const float *u, float *h1, float *h2, float *z, int quant)
+{
+    const int64_t M = (int64_t)nx * ny * nz * batch;
+    float *w1 = qcopy(W1, 16 * 27, quant), *w2 = qcopy(W2, 16 * 16 * 27, quant), *w3 = qcopy(W3, 16 * 27, quant);
+    float *uq = qcopy(u, M, quant);
+    double *acc = (double *)malloc(sizeof(double) * (size_t)M * 16);
+    conv3(nx, ny, nz, batch, 1, 16, w1, 0, uq, acc);
+    for (int64_t i = 0; i < M * 16; ++i) h1[i] = q_(acc[i] > 0 ? (float)acc[i] : 0.0f, quant);
+    conv3(nx, ny, nz, batch, 16, 16, w2, 0, h1, acc);
+    for (int64_t i = 0; i < M * 16; ++i) h2[i] = q_(acc[i] > 0 ? (float)acc[i] : 0.0f, quant);
+    conv3(nx, ny, nz, batch, 16, 1, w3, 0, h2, acc);
+    for (int64_t i = 0; i < M; ++i) z[i] = (float)acc[i];
+    free(acc); free(uq); free(w1); free(w2); free(w3);
+}
+
+void oracle_cnn_backward(int nx, int ny, int nz, int batch, const float *W1, const float *W2,
+                         const float *W3, const float *u, const float *h1, const float *h2,
+                         const float *dz, float *dW1, float *dW2, float *dW3, int quant)
+{
+    const int64_t M = (int64_t)nx * ny * nz * batch;
+    float *w2 = qcopy(W2, 16 * 16 * 27, quant), *w3 = qcopy(W3, 16 * 27, quant);
+    float *uq = qcopy(u, M, quant), *dzq = qcopy(dz, M, quant);
+    double *acc = (double *)malloc(sizeof(double) * (size_t)M * 16);
+    float *dh2 = (float *)malloc(sizeof(float) * (size_t)M * 16);
+    float *dh1 = (float *)malloc(sizeof(float) * (size_t)M * 16);
+    double g3[16 * 27] = {0}, g2[16 * 16 * 27] = {0}, g1[16 * 27] = {0};
+    conv3_wgrad(nx, ny, nz, batch, 16, 1, dzq, h2, g3);
+    conv3(nx, ny, nz, batch, 1, 16, w3, 1, dzq, acc); /* W3 as [co=1][ci=16] -> flipped [16][1] */
+    for (int64_t i = 0; i < M * 16; ++i) dh2[i] = h2[i] > 0 ? q_((float)acc[i], quant) : 0.0f;
+    conv3_wgrad(nx, ny, nz, batch, 16, 16, dh2, h1, g2);
+    conv3(nx, ny, nz, batch, 16, 16, w2, 1, dh2, acc);
+    for (int64_t i = 0; i < M * 16; ++i) dh1[i] = h1[i] > 0 ? q_((float)acc[i], quant) : 0.0f;
+    conv3_wgrad(nx, ny, nz, batch, 1, 16, dh1, uq, g1);
+    for (int i = 0; i < 16 * 27; ++i) dW3[i] = (float)g3[i];
+    for (int i = 0; i < 16 * 16 * 27; ++i) dW2[i] = (float)g2[i];
+    for (int i = 0; i < 16 * 27; ++i) dW1[i] = (float)g1[i];
+    free(acc); free(dh1); free(dh2); free(uq); free(dzq); free(w2); free(w3);
+}

@@ -62,6 +62,22 @@ void oracle_xstar_sines(int nx, int ny, int nz, uint64_t seed, double *xstar);
 void oracle_xstar_uniform(int64_t n, uint64_t seed, double *xstar);
 void oracle_batch_dims(int count, int lo, int hi, uint64_t seed, int32_t *dims);
 
+/* ---- learned preconditioner (no reference counterpart) -----------------------------------
+ * 3-layer bias-free 3x3x3 CNN on an nx*ny*nz grid (ix fastest), zero padding outside the grid:
+ *   h1 = relu(conv(u; W1[16][27])), h2 = relu(conv(h1; W2[16][16][27])), z = conv(h2; W3[16][27])
+ * conv(in; W)[co](v) = sum_{ci,t} W[co][ci][t] * in[ci](v + off(t)),
+ * off(t) = (t%3-1, (t/3)%3-1, t/9-1) in (x, y, z).  Activations are channel-last
+ * ([batch][voxel][16]).  Sums are formed in double.  quant != 0 applies the GPU's storage
+ * precisions: u, weights, h1, h2 (and dz, dh2, dh1 in the backward pass) rounded to bfloat16
+ * (round to nearest even); z and weight gradients are float. */
+float oracle_bf16(float v);
+void oracle_cnn_forward(int nx, int ny, int nz, int batch, const float *W1, const float *W2,
+                        const float *W3, const float *u, float *h1, float *h2, float *z, int quant);
+/* Given dz = dL/dz, the weight gradients dW1, dW2, dW3 (same layouts as W). */
+void oracle_cnn_backward(int nx, int ny, int nz, int batch, const float *W1, const float *W2,
+                         const float *W3, const float *u, const float *h1, const float *h2,
+                         const float *dz, float *dW1, float *dW2, float *dW3, int quant);
+
 #ifdef __cplusplus
 }
 #endif

@@ -71,6 +71,13 @@ def lib() -> ctypes.CDLL:
         L.oracle_xstar_uniform.argtypes = [ctypes.c_int64, ctypes.c_uint64, P]
         L.oracle_batch_dims.restype = None
         L.oracle_batch_dims.argtypes = [I, I, I, ctypes.c_uint64, P]
+        F = ctypes.c_float
+        L.oracle_bf16.restype = F
+        L.oracle_bf16.argtypes = [F]
+        L.oracle_cnn_forward.restype = None
+        L.oracle_cnn_forward.argtypes = [I, I, I, I, P, P, P, P, P, P, P, I]
+        L.oracle_cnn_backward.restype = None
+        L.oracle_cnn_backward.argtypes = [I, I, I, I, P, P, P, P, P, P, P, P, P, P, I]
         _lib = L
     return _lib
 
@@ -177,6 +184,45 @@ def batch_dims(count, lo, hi, seed) -> np.ndarray:
     out = np.zeros(3 * count, dtype=np.int32)
     lib().oracle_batch_dims(count, lo, hi, seed, _p(out))
     return out.reshape(count, 3)
+
+
+NW1, NW2, NW3 = 16 * 27, 16 * 16 * 27, 16 * 27
+
+
+def split_weights(w):
+    w = np.ascontiguousarray(w, dtype=np.float32)
+    return w[:NW1].copy(), w[NW1:NW1 + NW2].copy(), w[NW1 + NW2:].copy()
+
+
+def cnn_forward(w, u, dims, batch=1, quant=True):
+    """Returns (z, h1, h2) of the 3-layer network (hpccg_oracle.h)."""
+    W1, W2, W3 = split_weights(w)
+    nx, ny, nz = dims
+    V = nx * ny * nz * batch
+    u = np.ascontiguousarray(u, dtype=np.float32)
+    h1, h2, z = np.zeros(V * 16, np.float32), np.zeros(V * 16, np.float32), np.zeros(V, np.float32)
+    lib().oracle_cnn_forward(nx, ny, nz, batch, _p(W1), _p(W2), _p(W3), _p(u), _p(h1), _p(h2), _p(z),
+                             1 if quant else 0)
+    return z, h1, h2
+
+
+def cnn_backward(w, u, dz, dims, batch=1, quant=True):
+    W1, W2, W3 = split_weights(w)
+    z, h1, h2 = cnn_forward(w, u, dims, batch, quant)
+    nx, ny, nz = dims
+    dW1, dW2, dW3 = np.zeros(NW1, np.float32), np.zeros(NW2, np.float32), np.zeros(NW3, np.float32)
+    u = np.ascontiguousarray(u, dtype=np.float32)
+    dz = np.ascontiguousarray(dz, dtype=np.float32)
+    lib().oracle_cnn_backward(nx, ny, nz, batch, _p(W1), _p(W2), _p(W3), _p(u), _p(h1), _p(h2), _p(dz),
+                              _p(dW1), _p(dW2), _p(dW3), 1 if quant else 0)
+    return np.concatenate([dW1, dW2, dW3])
+
+
+def bf16(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32)
 
 
 def ref_binary(variant: str, name: str) -> str | None:

@@ -55,8 +55,8 @@ def test_cg_mixed_reaches_fp64_true_residual(ctx, grid, seed):
     e_mixed = float(np.max(np.abs(x - xs)))
     e_fp64 = float(np.max(np.abs(xo - xs)))
     assert e_mixed <= 10 * e_fp64 + 1e-12, (e_mixed, e_fp64)
-    # replacement points carry a true residual; Prepare's is ||b||
-    assert ht[0] == pytest.approx(normb, rel=1e-12)
+    # replacement points carry a true residual; Prepare's is ||b|| (of b - A*0 stored in FP32)
+    assert ht[0] == pytest.approx(normb, rel=1e-7)
     assert all(ht[j] > 0 for j in range(25, nit, 25))
 
 
