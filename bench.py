@@ -85,6 +85,9 @@ def main():
     ap.add_argument("--grid", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=20)
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--retrain", action="store_true", help="train the preconditioner even if weights are stored")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -93,6 +96,7 @@ def main():
 
     import numpy as np
     import hpccg_amd
+    from hpccg_amd import learned
 
     n1 = args.grid
     grid = (n1, n1, n1)
@@ -100,6 +104,15 @@ def main():
     os.makedirs(log_dir, exist_ok=True)
 
     ctx = hpccg_amd.Context(0, hpccg_amd.DOT_TREE)
+    # learned preconditioner: stored weights (trained by hpccg_amd.learned on this GPU), or
+    # trained here; either way outside the timed region and reported separately
+    train_info = None
+    weights = None if (args.no_learned or args.retrain) else learned.load()
+    if not args.no_learned and weights is None:
+        weights, train_info = learned.train(ctx, log=log)
+        learned.save(weights, os.path.join(log_dir, "precond_weights.npy"), train_info)
+    if weights is not None:
+        ctx.cnn_set_weights(weights)
     t0 = time.perf_counter()
     ctx.generate(*grid)
     info = ctx.matrix_info()
@@ -131,7 +144,15 @@ def main():
         return ctx.cg_mixed_dev(b, x, max_iter=2000, tolerance=REL_TOL * normb, replace_every=25,
                                 want_hist=False)
 
+    def run_learned():
+        x.upload(zeros)
+        r = ctx.fcg_learned_dev(grid, b, x, max_iter=2000, tolerance=REL_TOL * normb, replace_every=25)
+        r["niters"] = r["fcg_iters"] + r["cg_iters"]
+        return r
+
     solvers = {"fp64_cg": run_fp64, "mixed_cg": run_mixed}
+    if weights is not None:
+        solvers["learned_mixed_fcg"] = run_learned
     results = {}
     for name, fn in solvers.items():
         for _ in range(args.warmup):
@@ -146,6 +167,9 @@ def main():
         xerr = float(np.max(np.abs(x.numpy() - xs_h)))
         results[name] = {"s_wall": sum(times) / len(times), "s_device": sum(dev_ms) / len(dev_ms) / 1e3,
                          "niters": res["niters"], "true_relres": rel, "xerr_max": xerr}
+        if "fell_back" in res:
+            results[name].update(fcg_iters=res["fcg_iters"], fallback_cg_iters=res["cg_iters"],
+                                 fell_back=res["fell_back"])
         log(f"{name}: {results[name]}")
 
     # headline: the fastest method whose answer meets 1e-10 in the FP64 true residual
@@ -177,6 +201,35 @@ def main():
     log(f"spmv {spmv_ms:.4f} ms -> {achieved:.0f} GB/s ({100 * achieved / HBM_PEAK_GBS:.1f}% of 8 TB/s); "
         f"copy {roofline['copy_GBs']} GB/s")
 
+    # batched solves (64 systems, extents ~ U{16..96}, x* ~ U(-1,1), to 1e-8 each, one graph launch)
+    batched = None
+    if not args.no_batched:
+        dims = hpccg_amd.batch_dims(64, 16, 96, 2024)
+        binfo = ctx.batch_setup(dims)
+        off = binfo["row_off"]
+        xs_b = np.concatenate([hpccg_amd.xstar_uniform(int(off[s + 1] - off[s]), 5000 + s) for s in range(64)])
+        xb = hpccg_amd.DeviceArray.from_numpy(ctx, xs_b.astype(np.float32))
+        bb32 = hpccg_amd.DeviceArray(ctx, binfo["rows"], np.float32)
+        ctx.batch_spmv_dev(xb, bb32)  # b = A x* per system (FP64 sums of FP16 values x FP32 x*)
+        bb = hpccg_amd.DeviceArray.from_numpy(ctx, bb32.numpy().astype(np.float64))
+        xbd = hpccg_amd.DeviceArray.from_numpy(ctx, np.zeros(binfo["rows"]))
+        t_tune = time.perf_counter()
+        best, var_ms = ctx.batch_autotune(bb, xbd, probe_iters=8)
+        t_tune = time.perf_counter() - t_tune
+        ctx.batch_solve_dev(bb, xbd, rel_tol=1e-8, max_iter=400)  # capture + warm
+        xbd.upload(np.zeros(binfo["rows"]))
+        t = time.perf_counter()
+        res = ctx.batch_solve_dev(bb, xbd, rel_tol=1e-8, max_iter=400)
+        t = time.perf_counter() - t
+        variants = ctx.batch_variants()
+        batched = {"systems": 64, "rows": binfo["rows"], "nnz": binfo["nnz"], "s_wall": round(t, 6),
+                   "s_device": round(res["ms"] / 1e3, 6), "converged": int(res["converged"].sum()),
+                   "max_relres": float(res["relres"].max()), "iters_min_max": [int(res["niters"].min()),
+                                                                               int(res["niters"].max())],
+                   "autotune_s": round(t_tune, 3), "variant": {"rows": variants[best][0], "wg_per_cu": variants[best][1]},
+                   "variant_ms": [round(v, 4) for v in var_ms.tolist()]}
+        log(f"batched: {batched}")
+
     head = results[headline]
     cpu = None
     if not args.no_cpu:
@@ -201,6 +254,8 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "solvers": results,
+        "learned_training": train_info,
+        "batched": batched,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

@@ -846,11 +846,18 @@ static int finish_matrix_setup(hpccg_ctx *c, const int32_t *rowindex_host)
     c->graphs.clear();
     for (auto &g : c->mix.graphs) hipGraphExecDestroy(g.second);
     c->mix.graphs.clear();
+    invalidate_learned_graphs(c);
     return 0;
 }
 
+// Every captured graph of every path holds the matrix addresses: drop them with the matrix.
 static void free_matrix(hpccg_ctx *c)
 {
+    for (auto &g : c->graphs) hipGraphExecDestroy(g.second);
+    c->graphs.clear();
+    for (auto &g : c->mix.graphs) hipGraphExecDestroy(g.second);
+    c->mix.graphs.clear();
+    invalidate_learned_graphs(c);
     hipFree(c->A.rowindex);
     hipFree(c->A.colindex);
     hipFree(c->A.values);

@@ -164,6 +164,7 @@ static int ensure_act(hpccg_ctx *c, int64_t vox, int wg)
         N->cap_vox = vox;
         if (N->train_graph) hipGraphExecDestroy(N->train_graph);
         N->train_graph = nullptr;
+        invalidate_learned_graphs(c);  // captured FCG graphs hold the old activation buffers
     }
     if (wg > N->partials_wg) {
         hipFree(N->partials);
@@ -350,13 +351,16 @@ __global__ __launch_bounds__(256) void k_gather(const float *__restrict__ R, con
     }
 }
 
-// Adam (Kingma & Ba 2015) with a cosine learning-rate schedule over `total` steps; repacks are
-// launched after it.  Advances the device step counter.
+// Adam (Kingma & Ba 2015) with warm-up and a cosine learning-rate schedule over `total` steps;
+// the weight repack is launched after it.
 __global__ void k_adam(float *__restrict__ W, float *__restrict__ m, float *__restrict__ v,
                        const float *__restrict__ g, int n, float lr0, int total, int *step)
 {
     const int t = *step + 1;
-    const float lr = lr0 * 0.5f * (1.0f + cosf(3.14159265f * (float)min(t - 1, total) / (float)total));
+    // linear warm-up over the first 5% of the steps, then cosine decay to zero
+    const int warm = max(1, total / 20);
+    const float lr = t <= warm ? lr0 * (float)t / (float)warm
+                               : lr0 * 0.5f * (1.0f + cosf(3.14159265f * (float)(t - warm) / (float)max(1, total - warm)));
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
     const float c1 = 1.0f - powf(b1, (float)t), c2 = 1.0f - powf(b2, (float)t);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -597,10 +601,19 @@ struct FcgWs {
     double *hist = nullptr;
     int hist_cap = 0;
     uint16_t *h1 = nullptr, *h2 = nullptr;
+    int gdims[3] = {0, 0, 0};  // grid the cached graphs were captured for
     std::map<GraphKey, hipGraphExec_t> graphs;
 };
 
 static FcgWs *fcg_of(hpccg_ctx *c) { return (FcgWs *)c->ext[2]; }
+
+void invalidate_learned_graphs(hpccg_ctx *c)
+{
+    if (FcgWs *F = fcg_of(c)) {
+        for (auto &g : F->graphs) hipGraphExecDestroy(g.second);
+        F->graphs.clear();
+    }
+}
 
 void free_learned(hpccg_ctx *c)
 {
@@ -641,7 +654,8 @@ extern "C" {
 
 int hpccg_cnn_num_weights(void) { return NWALL; }
 
-// He-normal initial weights (Box-Muller on SplitMix64 draws), Adam state reset.
+// Initial weights uniform in +-1/sqrt(fan_in) (fan_in = 27 * input channels; SplitMix64 draws),
+// Adam state reset.  Small enough that no ReLU layer starts saturated.
 int hpccg_cnn_init(hpccg_ctx *c, uint64_t seed)
 {
     HPCCG_CHECK_ARG(c, "null context");
@@ -656,14 +670,9 @@ int hpccg_cnn_init(hpccg_ctx *c, uint64_t seed)
         z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
         return ((z ^ (z >> 31)) >> 11) * 0x1.0p-53;
     };
-    auto normal = [&]() {
-        const double u1 = std::max(next(), 1e-300), u2 = next();
-        return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
-    };
-    const double s1 = std::sqrt(2.0 / 27.0), s2 = std::sqrt(2.0 / (16.0 * 27.0)), s3 = std::sqrt(1.0 / (16.0 * 27.0));
-    for (int i = 0; i < NW1; ++i) w[i] = (float)(s1 * normal());
-    for (int i = 0; i < NW2; ++i) w[NW1 + i] = (float)(s2 * normal());
-    for (int i = 0; i < NW3; ++i) w[NW1 + NW2 + i] = (float)(s3 * normal());
+    const double b1 = 1.0 / std::sqrt(27.0), b2 = 1.0 / std::sqrt(16.0 * 27.0);
+    for (int i = 0; i < NW1; ++i) w[i] = (float)(b1 * (2.0 * next() - 1.0));
+    for (int i = 0; i < NW2 + NW3; ++i) w[NW1 + i] = (float)(b2 * (2.0 * next() - 1.0));
     CnnNet *N = net_of(c);
     HPCCG_HIP_TRY(hipMemcpy(N->W, w.data(), NWALL * 4, hipMemcpyHostToDevice));
     std::vector<float> zero(NWALL, 0.f);
@@ -980,6 +989,12 @@ int hpccg_fcg_learned_dev(hpccg_ctx *c, int32_t nx, int32_t ny, int32_t nz, cons
     HPCCG_TRY(ensure_act(c, g.nvox(), 1));
     if (!fcg_of(c)) c->ext[2] = new FcgWs();
     FcgWs *F = fcg_of(c);
+    if (F->gdims[0] != nx || F->gdims[1] != ny || F->gdims[2] != nz) {  // the grid is baked into the graphs
+        invalidate_learned_graphs(c);
+        F->gdims[0] = nx;
+        F->gdims[1] = ny;
+        F->gdims[2] = nz;
+    }
     if (F->n < n) {
         for (auto &gg : F->graphs) hipGraphExecDestroy(gg.second);
         F->graphs.clear();

@@ -92,6 +92,8 @@ namespace hpccg {
 void free_mixed(hpccg_ctx *c);
 void free_batched(hpccg_ctx *c);
 void free_learned(hpccg_ctx *c);
+// drop the learned path's captured graphs (they hold matrix and activation addresses)
+void invalidate_learned_graphs(hpccg_ctx *c);
 int ensure_values16(hpccg_ctx *c);
 int cg_device_fp64(hpccg_ctx *c, const double *b, double *x, int max_iter, double tol, int *niters,
                    double *normr, double *hist, double *elapsed_ms);

@@ -1,0 +1,82 @@
+"""Learned preconditioner workflow: training-set generation from plain CG, training with the
+library's own MFMA kernels, and weight files.
+
+The network and its kernels live in libhpccg_hip (csrc/cnn_kernels.hpp, csrc/cnn.hip); this
+module only sequences the C-ABI calls and stores weights (7776 float32 values:
+W1[16][27] | W2[16][16][27] | W3[16][27], tap t = 9*(dz+1) + 3*(dy+1) + (dx+1)).
+"""
+from __future__ import annotations
+
+import json
+import os
+import time
+
+import numpy as np
+
+DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+DEFAULT_WEIGHTS = os.path.join(DATA_DIR, "precond_weights.npy")
+
+# Training recipe: 20k (residual, error) pairs from plain FP64 CG runs on 64^3 grids with
+# sine + noise exact solutions (one random 24^3 window per pair), Adam with warm-up and cosine
+# decay, random reflections and sign flips of each window.
+# Several initialisations are trained on the same pairs and the one whose flexible CG needs the
+# fewest iterations on a validation system (its own seed, never used for evaluation) is kept.
+RECIPE = {"n": 64, "npairs": 20000, "crop": 16, "data_seed": 21, "steps": 3000, "batch": 32, "lr": 2e-3,
+          "candidates": 6, "first_seed": 3, "val_n": 40, "val_seed": 4242}
+
+
+def validation_iters(ctx, weights, n, seed, max_iter=400):
+    """Flexible-CG iterations to 1e-10 on an n^3 sines system (fresh seed), or max_iter when the
+    solve fell back to plain CG."""
+    from . import _lib
+    ctx.cnn_set_weights(weights)
+    ctx.generate(n, n, n)
+    xs = _lib.DeviceArray.from_numpy(ctx, _lib.xstar_sines(n, n, n, seed))
+    b = _lib.DeviceArray(ctx, n ** 3)
+    ctx.spmv_dev(xs, b)
+    normb = float(np.sqrt(ctx.ddot_dev(b, b)))
+    x = _lib.DeviceArray.from_numpy(ctx, np.zeros(n ** 3))
+    r = ctx.fcg_learned_dev((n, n, n), b, x, max_iter=max_iter, tolerance=1e-10 * normb)
+    return max_iter if r["fell_back"] else r["fcg_iters"]
+
+
+def train(ctx, recipe=None, log=print):
+    """Builds the training set, trains `candidates` initialisations on the context's GPU and keeps
+    the best on the validation system.  Returns (weights, info); the context's matrix is left
+    replaced by the validation grid."""
+    r = dict(RECIPE, **(recipe or {}))
+    t0 = time.perf_counter()
+    ctx.cnn_make_dataset(n=r["n"], npairs=r["npairs"], crop=r["crop"], seed=r["data_seed"])
+    t1 = time.perf_counter()
+    best, cands = None, []
+    for k in range(r["candidates"]):
+        seed = r["first_seed"] + k
+        ctx.cnn_init(seed=seed)
+        loss, ms = ctx.cnn_train(steps=r["steps"], batch=r["batch"], lr=r["lr"], seed=seed + 1000)
+        w = ctx.cnn_get_weights()
+        it = validation_iters(ctx, w, r["val_n"], r["val_seed"])
+        cands.append({"seed": seed, "loss_last": float(loss[-50:].mean()), "train_device_ms": ms,
+                      "val_iters": it})
+        log(f"  candidate {k}: loss {loss[-50:].mean():.4f}, validation FCG iterations {it}")
+        if best is None or it < best[0]:
+            best = (it, w)
+    t2 = time.perf_counter()
+    info = {"recipe": r, "dataset_s": t1 - t0, "train_and_select_s": t2 - t1, "candidates": cands}
+    log(f"learned preconditioner: dataset {info['dataset_s']:.2f} s, training + selection "
+        f"{info['train_and_select_s']:.2f} s, best validation iterations {best[0]}")
+    return best[1], info
+
+
+def save(weights, path=DEFAULT_WEIGHTS, info=None):
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    np.save(path, np.asarray(weights, dtype=np.float32), allow_pickle=False)
+    if info is not None:
+        with open(os.path.splitext(path)[0] + ".json", "w") as f:
+            json.dump(info, f, indent=1)
+
+
+def load(path=DEFAULT_WEIGHTS):
+    if not os.path.exists(path):
+        return None
+    w = np.load(path, allow_pickle=False)
+    return np.ascontiguousarray(w, dtype=np.float32)

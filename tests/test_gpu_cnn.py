@@ -164,3 +164,32 @@ def test_fcg_falls_back_to_plain_cg(ctx, kind):
     rel = math.sqrt(oracle.ddot1(b - oracle.spmv(A, x))) / normb
     assert res["fell_back"], res
     assert rel <= 2e-10, rel
+
+
+def test_solver_graph_caches_survive_matrix_reuploads(ctx):
+    """Regression: captured graphs hold matrix and workspace addresses; re-uploading matrices of
+    other sizes and allocating fresh vectors (which may land on freed addresses) must never
+    replay a graph built for the old buffers.  Identity network: FCG = plain CG."""
+    w = np.zeros(oracle.NW1 + oracle.NW2 + oracle.NW3, np.float32)
+    c = 13
+    w[0 * 27 + c], w[1 * 27 + c] = 1, -1
+    w[oracle.NW1 + (0 * 16 + 0) * 27 + c] = w[oracle.NW1 + (1 * 16 + 1) * 27 + c] = 1
+    w[oracle.NW1 + oracle.NW2 + 0 * 27 + c], w[oracle.NW1 + oracle.NW2 + 1 * 27 + c] = 1, -1
+    ctx.cnn_set_weights(w)
+    for n in (12, 16, 12, 20, 16):
+        A, xs, b = _system(n, 40 + n)
+        normb = math.sqrt(oracle.ddot1(b))
+        ctx.upload(A.rowindex, A.colindex, A.values)
+        for solver in ("fp64", "mixed", "fcg"):
+            bd = hpccg_amd.DeviceArray.from_numpy(ctx, b)
+            xd = hpccg_amd.DeviceArray.from_numpy(ctx, np.zeros(A.n))
+            if solver == "fp64":
+                ctx.cg_dev(bd, xd, max_iter=500, tolerance=1e-10 * normb)
+            elif solver == "mixed":
+                ctx.cg_mixed_dev(bd, xd, max_iter=500, tolerance=1e-10 * normb)
+            else:
+                res = ctx.fcg_learned_dev((n, n, n), bd, xd, max_iter=500, tolerance=1e-10 * normb)
+                assert not res["fell_back"]
+            rel = math.sqrt(oracle.ddot1(b - oracle.spmv(A, xd.numpy()))) / normb
+            assert rel <= 2e-10, (n, solver, rel)
+            del bd, xd
