@@ -111,6 +111,11 @@ def main():
     if not args.no_learned and weights is None:
         weights, train_info = learned.train(ctx, log=log)
         learned.save(weights, os.path.join(log_dir, "precond_weights.npy"), train_info)
+    elif weights is not None:
+        stored = os.path.splitext(learned.DEFAULT_WEIGHTS)[0] + ".json"
+        if os.path.exists(stored):
+            with open(stored) as f:
+                train_info = dict(json.load(f), source="stored weights (trained on the GPU by hpccg_amd.learned.train)")
     if weights is not None:
         ctx.cnn_set_weights(weights)
     t0 = time.perf_counter()
