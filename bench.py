@@ -7,7 +7,8 @@ inputs: b = A x*, x* = 8 random low-frequency 3-D sines + U(-0.05, 0.05) noise (
 The stopping test is the reference's (recursive residual, numerics.F90:109), with tolerance
 1e-10 * ||b||; the FP64 true residual ||b - A x|| / ||b|| of every result is reported.
 
-    python bench.py [--steps K] [--warmup W] [--grid N] [--no-cpu] [--gpus 1]
+    python bench.py [--steps K] [--warmup W] [--grid N] [--no-cpu] [--no-learned] [--no-batched]
+                    [--spmv-only] [--gpus 1]
 
 Prints one JSON line (last line of stdout).
 """
@@ -74,7 +75,34 @@ def cpu_baseline(grid, b, gpu_niters, log_dir):
     return {"value": total, "unit": "s", "cores": cores, "kind": "reference",
             "sample": f"full solve of the same {nx}x{ny}x{nz} system to 1e-10 by the reference cg "
                       f"(MPI/src, amdflang -O3 -fopenmp -D_OMP), its TIMER: Total; {niters} iterations "
-                      f"(GPU fp64: {gpu_niters}); process wall incl. generate_matrix {wall:.1f} s"}
+                      f"(GPU FP64 CG: {gpu_niters}); process wall incl. generate_matrix {wall:.1f} s"}
+
+
+def measure_roofline(ctx, xs, n, nnz, n1, reps):
+    """Roofline of the dominant kernel: the FP64 CSR SpMV as the CG loop launches it, timed with
+    HIP events on the context's stream, plus a plain copy of similar size as the HBM calibration."""
+    import hpccg_amd
+    y = hpccg_amd.DeviceArray(ctx, n)
+    spmv_ms = ctx.time_spmv(xs, y, reps=reps)
+    B = spmv_bytes(n, nnz)
+    achieved = B / (spmv_ms * 1e-3) / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", f"spmv_fp64_{n1}_traffic.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get("traffic_bytes")
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": "k_cg_spmv<64,true> (FP64 CSR SpMV + p.Ap epilogue)",
+                "algorithmic_bytes": B, "avg_ms": round(spmv_ms, 5)}
+    cb = min(B // 2, 2 ** 31) // 16 * 16
+    ca = hpccg_amd.DeviceArray(ctx, cb // 8)
+    cbuf = hpccg_amd.DeviceArray(ctx, cb // 8)
+    copy_ms = ctx.time_copy(cbuf, ca, reps=20)
+    roofline["copy_GBs"] = round(2 * cb / (copy_ms * 1e-3) / 1e9, 1)
+    log(f"spmv {spmv_ms:.4f} ms -> {achieved:.0f} GB/s ({100 * achieved / HBM_PEAK_GBS:.1f}% of 8 TB/s); "
+        f"copy {roofline['copy_GBs']} GB/s")
+    return roofline
 
 
 def main():
@@ -88,6 +116,8 @@ def main():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--retrain", action="store_true", help="train the preconditioner even if weights are stored")
+    ap.add_argument("--spmv-only", action="store_true",
+                    help="only build the matrix and time the SpMV and the copy (for counter runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -107,6 +137,8 @@ def main():
     # learned preconditioner: stored weights (trained by hpccg_amd.learned on this GPU), or
     # trained here; either way outside the timed region and reported separately
     train_info = None
+    if args.spmv_only:
+        args.no_learned = True
     weights = None if (args.no_learned or args.retrain) else learned.load()
     if not args.no_learned and weights is None:
         weights, train_info = learned.train(ctx, log=log)
@@ -132,6 +164,10 @@ def main():
     r = hpccg_amd.DeviceArray(ctx, n)
     zeros = np.zeros(n)
     log(f"setup {time.perf_counter() - t0:.1f} s: grid {grid}, n={n}, nnz={nnz}, ||b||={normb:.6e}")
+    if args.spmv_only:
+        print(json.dumps({"roofline": measure_roofline(ctx, xs, n, nnz, n1, args.spmv_reps)}), flush=True)
+        ctx.close()
+        return
 
     def true_relres():
         ctx.spmv_dev(x, r)
@@ -183,28 +219,7 @@ def main():
     for k in results:
         results[k]["speedup_vs_fp64_cg"] = round(results["fp64_cg"]["s_wall"] / results[k]["s_wall"], 3)
 
-    # roofline of the dominant kernel: the FP64 CSR SpMV as the CG loop launches it
-    y = hpccg_amd.DeviceArray(ctx, n)
-    spmv_ms = ctx.time_spmv(xs, y, reps=args.spmv_reps)
-    B = spmv_bytes(n, nnz)
-    achieved = B / (spmv_ms * 1e-3) / 1e9
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", f"spmv_fp64_{n1}_traffic.json")
-    if os.path.exists(tf):
-        with open(tf) as f:
-            traffic = json.load(f).get("traffic_bytes")
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "k_cg_spmv<128,true> (FP64 CSR SpMV + p.Ap epilogue)",
-                "algorithmic_bytes": B, "avg_ms": round(spmv_ms, 5)}
-    # calibration: a plain copy of the same byte count
-    cb = min(B // 2, 2 ** 31) // 16 * 16
-    ca = hpccg_amd.DeviceArray(ctx, cb // 8)
-    cbuf = hpccg_amd.DeviceArray(ctx, cb // 8)
-    copy_ms = ctx.time_copy(cbuf, ca, reps=20)
-    roofline["copy_GBs"] = round(2 * cb / (copy_ms * 1e-3) / 1e9, 1)
-    log(f"spmv {spmv_ms:.4f} ms -> {achieved:.0f} GB/s ({100 * achieved / HBM_PEAK_GBS:.1f}% of 8 TB/s); "
-        f"copy {roofline['copy_GBs']} GB/s")
+    roofline = measure_roofline(ctx, xs, n, nnz, n1, args.spmv_reps)
 
     # batched solves (64 systems, extents ~ U{16..96}, x* ~ U(-1,1), to 1e-8 each, one graph launch)
     batched = None
@@ -238,7 +253,7 @@ def main():
     head = results[headline]
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(grid, b_h, head["niters"], log_dir)
+        cpu = cpu_baseline(grid, b_h, results["fp64_cg"]["niters"], log_dir)
         log(f"cpu baseline: {cpu}")
 
     out = {

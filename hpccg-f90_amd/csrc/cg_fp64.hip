@@ -54,7 +54,7 @@ __device__ __forceinline__ double spmv_tile(const int32_t *__restrict__ rowptr,
 
 // y = A x  (standalone spmv; no reduction)
 template <int ROWS>
-__global__ __launch_bounds__(ROWS) void k_spmv(const int32_t *__restrict__ rowptr,
+__global__ __launch_bounds__(ROWS) HPCCG_SPMV_OCC void k_spmv(const int32_t *__restrict__ rowptr,
                                                const int32_t *__restrict__ col,
                                                const double *__restrict__ val,
                                                const double *__restrict__ x, double *__restrict__ y,
@@ -99,7 +99,7 @@ __device__ __forceinline__ void cg_after_rr(CgState *st, double *rr_hist, int k,
 // ---- CG kernels ----------------------------------------------------------------------------
 // Ap = A p and (TREE) p.Ap -> alpha                       numerics.F90:133-138
 template <int ROWS, bool TREE>
-__global__ __launch_bounds__(ROWS) void k_cg_spmv(const int32_t *__restrict__ rowptr,
+__global__ __launch_bounds__(ROWS) HPCCG_SPMV_OCC void k_cg_spmv(const int32_t *__restrict__ rowptr,
                                                   const int32_t *__restrict__ col,
                                                   const double *__restrict__ val,
                                                   const double *__restrict__ p,
@@ -125,7 +125,7 @@ __global__ __launch_bounds__(ROWS) void k_cg_spmv(const int32_t *__restrict__ ro
 
 // Prepare: Ap = A x (p = x, numerics.F90:92-96)
 template <int ROWS>
-__global__ __launch_bounds__(ROWS) void k_cg_prep_spmv(const int32_t *__restrict__ rowptr,
+__global__ __launch_bounds__(ROWS) HPCCG_SPMV_OCC void k_cg_prep_spmv(const int32_t *__restrict__ rowptr,
                                                        const int32_t *__restrict__ col,
                                                        const double *__restrict__ val,
                                                        const double *__restrict__ x,
@@ -1163,7 +1163,7 @@ int hpccg_time_copy(hpccg_ctx *c, void *dst, const void *src, int64_t bytes, int
 {
     HPCCG_CHECK_ARG(c && dst && src && reps > 0 && avg_ms && bytes % 16 == 0, "bad argument (bytes % 16 == 0)");
     const int64_t n16 = bytes / 16;
-    const int G = c->info.num_cu * 8;
+    const int G = c->info.num_cu * 4;  // 4 workgroups of 256 per CU: the fastest shape measured
     hipLaunchKernelGGL(k_copy16, dim3(G), dim3(256), 0, c->stream, (const u32x4 *)src, (u32x4 *)dst, n16);
     HPCCG_HIP_TRY(hipEventRecord(c->ev0, c->stream));
     for (int i = 0; i < reps; ++i)

@@ -282,11 +282,8 @@ __global__ __launch_bounds__(256) void k_loss_b(const float *__restrict__ wbuf, 
 {
     const int b = blockIdx.x / slabs, sl = blockIdx.x % slabs;
     const int64_t V = (int64_t)C * C * C;
-    double num = 0.0, den = 0.0;
-    for (int i = 0; i < slabs; ++i) {
-        num += lpart[2 * (b * slabs + i)];
-        den += lpart[2 * (b * slabs + i) + 1];
-    }
+    double den = 0.0;
+    for (int i = 0; i < slabs; ++i) den += lpart[2 * (b * slabs + i) + 1];
     const double inv = den > 0.0 ? 2.0 / den : 0.0;
     const float *wb = wbuf + b * V;
     const int z0 = (int)((int64_t)sl * C / slabs), z1 = (int)((int64_t)(sl + 1) * C / slabs);

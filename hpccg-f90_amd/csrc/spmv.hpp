@@ -11,6 +11,10 @@ constexpr int kTileRows = 64;  // rows per SpMV tile == threads per SpMV workgro
 constexpr int kVecThreads = 256;
 constexpr int kMaxGrid = 16384;
 
+// FP64 SpMV kernels: LDS (27 FP64 products per row, 13.8 KB per 64-row tile) admits 11
+// workgroups per CU, i.e. 3 waves per SIMD; cap the registers so they do not cut that to 2.
+#define HPCCG_SPMV_OCC __attribute__((amdgpu_waves_per_eu(3)))
+
 __device__ __forceinline__ double to_f64(double v) { return v; }
 __device__ __forceinline__ double to_f64(float v) { return (double)v; }
 __device__ __forceinline__ double to_f64(__half v) { return (double)__half2float(v); }
@@ -33,7 +37,11 @@ __device__ __forceinline__ __half ld_stream(const __half *p)
 // order.  VT = stored value type (double, or __half holding {27,-1} exactly), XT = vector
 // type.  With VT = double and XT = double the result is bitwise the reference's spmv.
 // Returns the row sum (FP64) of this thread's row, or 0 past the last row.
-template <int ROWS, typename VT, typename XT, int UNR = 16>
+//
+// The phase-1 loads are branch-free: a slot past the tile's last element re-reads that last
+// element (a valid address) and its product is dropped, so all UNR value and column loads of
+// a thread issue back to back.  UNR = 27 covers a full tile of the 27-point stencil in one pass.
+template <int ROWS, typename VT, typename XT, int UNR = 27>
 __device__ __forceinline__ double spmv_tile_rowsum(const int32_t *__restrict__ rowptr,
                                                    const int32_t *__restrict__ col,
                                                    const VT *__restrict__ val,
@@ -55,21 +63,26 @@ __device__ __forceinline__ double spmv_tile_rowsum(const int32_t *__restrict__ r
     }
     double sum = 0.0;
     if (end - base <= ROWS * 27) {
-        for (int k = base + t; k < end; k += UNR * ROWS) {
+        const int last = end - 1;  // only read when end > base
+        for (int k0 = base; k0 < end; k0 += UNR * ROWS) {
             VT v[UNR];
             int c[UNR];
 #pragma unroll
             for (int u = 0; u < UNR; ++u) {
-                const int kk = k + u * ROWS;
-                if (kk < end) {
-                    v[u] = ld_stream(val + kk);
-                    c[u] = ld_stream(col + kk);
-                }
+                const int kk = min(k0 + t + u * ROWS, last);
+                v[u] = ld_stream(val + kk);
+                c[u] = ld_stream(col + kk);
             }
+            // all gathers before the first LDS store: x is a generic pointer the compiler must
+            // assume may alias LDS, so an interleaved store would serialise the gathers
+            XT xg[UNR];
+#pragma unroll
+            for (int u = 0; u < UNR; ++u) xg[u] = x[c[u] - 1];
 #pragma unroll
             for (int u = 0; u < UNR; ++u) {
-                const int kk = k + u * ROWS;
-                if (kk < end) lds[kk - base] = to_f64(v[u]) * to_f64(x[c[u] - 1]);
+                const int kk = k0 + t + u * ROWS;
+                const double prod = to_f64(v[u]) * to_f64(xg[u]);
+                if (kk < end) lds[kk - base] = prod;
             }
         }
         __syncthreads();

@@ -17,7 +17,7 @@ DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
 DEFAULT_WEIGHTS = os.path.join(DATA_DIR, "precond_weights.npy")
 
 # Training recipe: 20k (residual, error) pairs from plain FP64 CG runs on 64^3 grids with
-# sine + noise exact solutions (one random 24^3 window per pair), Adam with warm-up and cosine
+# sine + noise exact solutions (one random 16^3 window per pair), Adam with warm-up and cosine
 # decay, random reflections and sign flips of each window.
 # Several initialisations are trained on the same pairs and the one whose flexible CG needs the
 # fewest iterations on a validation system (its own seed, never used for evaluation) is kept.

@@ -164,6 +164,51 @@ int hpccg_xstar_uniform(int64_t n, uint64_t seed, double *xstar);
 /* 3*count extents ~ U{lo..hi}: (nx,ny,nz) of each batched system */
 int hpccg_batch_dims(int32_t count, int32_t lo, int32_t hi, uint64_t seed, int32_t *dims);
 
+/* ---- learned preconditioner (no reference counterpart: the reference's cg is unpreconditioned,
+ * numerics.F90:41-162).  A 3-layer bias-free 3x3x3 convolutional network u -> z
+ * (1 -> 16 -> 16 -> 1 channels, ReLU after layers 1 and 2, zero padding at the grid faces),
+ * BF16 activations and weights, FP32 accumulation on MFMA.  Weights are FP32 masters,
+ * hpccg_cnn_num_weights() of them: W1[16][27], W2[16][16][27], W3[16][27], tap index
+ * t = (dz+1)*9 + (dy+1)*3 + (dx+1), voxel index (z*ny + y)*nx + x. -------------------------- */
+int hpccg_cnn_num_weights(void);
+/* Weights uniform in +-1/sqrt(fan_in) from a SplitMix64 stream; Adam state reset. */
+int hpccg_cnn_init(hpccg_ctx *ctx, uint64_t seed);
+int hpccg_cnn_set_weights(hpccg_ctx *ctx, const float *w);
+int hpccg_cnn_get_weights(hpccg_ctx *ctx, float *w);
+/* Forward pass on host arrays: u[batch][nvox] (rounded to BF16), h1/h2 = FP32 copies of the
+ * BF16 hidden activations [batch][nvox][16] (may be NULL), z[batch][nvox]. */
+int hpccg_cnn_forward(hpccg_ctx *ctx, int32_t nx, int32_t ny, int32_t nz, int32_t batch,
+                      const float *u, float *h1, float *h2, float *z);
+/* Forward + backward on host arrays for a given dL/dz: dW[hpccg_cnn_num_weights()]. */
+int hpccg_cnn_backward(hpccg_ctx *ctx, int32_t nx, int32_t ny, int32_t nz, int32_t batch,
+                       const float *u, const float *dz, float *dW);
+/* Training set: npairs (r_k, e_k) pairs from plain FP64 CG runs on n^3 grids (seeded
+ * x* = hpccg_xstar_sines), each a random crop^3 window scaled by 1/rms(r).  Replaces the
+ * context's matrix. */
+int hpccg_cnn_make_dataset(hpccg_ctx *ctx, int32_t n, int32_t npairs, int32_t crop, uint64_t seed);
+/* Pair i of the training set (crop^3 floats each); margins6 (may be NULL) = loss-mask margins. */
+int hpccg_cnn_dataset_pair(hpccg_ctx *ctx, int32_t i, float *r, float *e, int32_t *margins6);
+/* `steps` Adam steps (warm-up, cosine decay) of the A-norm error loss
+ * ||mask.A(z - e)||^2 / ||mask.A e||^2 on batches of random reflected crops; one captured graph
+ * per step.  loss_hist[steps] may be NULL; elapsed_ms = device time of all steps. */
+int hpccg_cnn_train(hpccg_ctx *ctx, int32_t steps, int32_t batch, float lr, uint64_t seed,
+                    float *loss_hist, double *elapsed_ms);
+/* z = rms(r) * net(r / rms(r)) on device arrays of one grid; inv_scale = 1/rms(r). */
+int hpccg_cnn_apply_dev(hpccg_ctx *ctx, int32_t nx, int32_t ny, int32_t nz, const float *r,
+                        float inv_scale, float *z);
+/* Flexible CG preconditioned by the network (FP16 matrix values, FP32 r/p/Ap/z, FP64 dots and
+ * x, true-residual replacement every replace_every iterations, stop on the FP64 true residual
+ * ||b - A x|| <= tolerance).  Every `window` iterations ||r|| must have dropped by min_factor,
+ * else (or on a breakdown) the current x is handed to FP64 CG (hpccg_cg_dev semantics) for the
+ * rest of the solve.  (nx,ny,nz) is the matrix's grid.  hist_rec[0..max_iter] (may be NULL)
+ * receives the recurrence ||r|| per FCG iteration, -1 past the last; elapsed_ms covers the
+ * whole solve. */
+int hpccg_fcg_learned_dev(hpccg_ctx *ctx, int32_t nx, int32_t ny, int32_t nz, const double *b,
+                          double *x, int32_t max_iter, double tolerance, int32_t replace_every,
+                          int32_t window, double min_factor, int32_t *fcg_iters,
+                          int32_t *cg_iters, int32_t *fell_back, double *true_normr,
+                          double *hist_rec, double *elapsed_ms);
+
 /* ---- measurement -------------------------------------------------------------------------- */
 /* Run `reps` back-to-back FP64 SpMVs y = A x on device arrays -- the kernel the CG loop
  * launches, with its p.Ap epilogue -- and return the mean kernel time in ms (HIP events on the

@@ -126,10 +126,14 @@ def _system(n, seed):
     return A, xs, oracle.spmv(A, xs)
 
 
-def test_fcg_learned_gain_and_accuracy(ctx, trained):
-    """The trained network cuts flexible-CG iterations below plain CG's on a fresh system,
-    and the answer meets the tolerance in the FP64 true residual (judged by the oracle)."""
-    ctx.cnn_set_weights(trained["w"])
+def test_fcg_learned_gain_and_accuracy(ctx):
+    """The shipped network (hpccg_amd/data, trained on 64^3 pairs and selected on a 40^3
+    validation system) cuts flexible-CG iterations well below plain CG's on a fresh 32^3
+    system, and the answer meets the tolerance in the FP64 true residual (judged by the oracle)."""
+    from hpccg_amd import learned
+    w = learned.load()
+    assert w is not None and w.size == oracle.NW1 + oracle.NW2 + oracle.NW3
+    ctx.cnn_set_weights(w)
     n = 32
     A, xs, b = _system(n, 777)
     normb = math.sqrt(oracle.ddot1(b))
@@ -142,7 +146,24 @@ def test_fcg_learned_gain_and_accuracy(ctx, trained):
     assert rel <= 1e-10, (rel, res)
     _, nit_plain, _, _, _ = oracle.cg(A, b, max_iter=500, tolerance=1e-10 * normb)
     assert not res["fell_back"], res
-    assert res["fcg_iters"] < nit_plain, (res["fcg_iters"], nit_plain)
+    assert res["fcg_iters"] <= 0.8 * nit_plain, (res["fcg_iters"], nit_plain)
+
+
+def test_training_pipeline_selects_a_network_that_beats_plain_cg(ctx):
+    """hpccg_amd.learned.train end to end at a reduced size (48^3 pairs, 4 initialisations):
+    the selected network needs fewer flexible-CG iterations than plain CG on the validation
+    system, without falling back."""
+    from hpccg_amd import learned
+    recipe = {"n": 48, "npairs": 4000, "crop": 16, "steps": 1500, "batch": 32, "candidates": 4,
+              "val_n": 32, "val_seed": 99}
+    w, info = learned.train(ctx, recipe, log=lambda *a: None)
+    best = min(c["val_iters"] for c in info["candidates"])
+    n = recipe["val_n"]
+    A = oracle.Problem(n, n, n)
+    b = oracle.spmv(A, oracle.xstar_sines(n, n, n, recipe["val_seed"]))
+    _, nit_plain, _, _, _ = oracle.cg(A, b, max_iter=500, tolerance=1e-10 * math.sqrt(oracle.ddot1(b)))
+    assert best < nit_plain, (info["candidates"], nit_plain)
+    assert learned.validation_iters(ctx, w, n, recipe["val_seed"]) == best
 
 
 @pytest.mark.parametrize("kind", ["zero", "stagnation"])
