@@ -67,11 +67,20 @@ __device__ __forceinline__ double spmv_tile_rowsum(const int32_t *__restrict__ r
         for (int k0 = base; k0 < end; k0 += UNR * ROWS) {
             VT v[UNR];
             int c[UNR];
+            if (sizeof(VT) == 8) {
+                // FP64 values: all column loads first (their gathers can start while the values
+                // are in flight; this order also needs fewer registers)
 #pragma unroll
-            for (int u = 0; u < UNR; ++u) {
-                const int kk = min(k0 + t + u * ROWS, last);
-                v[u] = ld_stream(val + kk);
-                c[u] = ld_stream(col + kk);
+                for (int u = 0; u < UNR; ++u) c[u] = ld_stream(col + min(k0 + t + u * ROWS, last));
+#pragma unroll
+                for (int u = 0; u < UNR; ++u) v[u] = ld_stream(val + min(k0 + t + u * ROWS, last));
+            } else {
+#pragma unroll
+                for (int u = 0; u < UNR; ++u) {
+                    const int kk = min(k0 + t + u * ROWS, last);
+                    v[u] = ld_stream(val + kk);
+                    c[u] = ld_stream(col + kk);
+                }
             }
             // all gathers before the first LDS store: x is a generic pointer the compiler must
             // assume may alias LDS, so an interleaved store would serialise the gathers
