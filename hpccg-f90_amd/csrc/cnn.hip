@@ -187,6 +187,21 @@ static void forward(hpccg_ctx *c, const float *in, const float *s_in, float *z, 
     conv<IN_BF16_16, OUT_F32_CH0>(c, N->h2, nullptr, N->pk_f3, z, nullptr, s_out, g, batch, skip);
 }
 
+// inference on one field: the fused kernel (bitwise equal to forward(), no activation traffic)
+static void forward_fused(hpccg_ctx *c, const float *in, const float *s_in, float *z, const float *s_out,
+                          const Grid3 &g, const int *skip = nullptr)
+{
+    CnnNet *N = net_of(c);
+    const int tx = (g.nx + FX - 1) / FX, ty = (g.ny + FY - 1) / FY;
+    // about two workgroups per CU (the LDS rings admit two), z-chunks of at least 8 planes
+    const int want = std::max(1, (2 * c->info.num_cu + tx * ty - 1) / (tx * ty));
+    const int nzc = std::max(1, std::min(want, (g.nz + 7) / 8));
+    const int zchunk = (g.nz + nzc - 1) / nzc;
+    const int chunks = (g.nz + zchunk - 1) / zchunk;
+    hipLaunchKernelGGL(k_fused_fwd, dim3(tx * ty * chunks), dim3(256), 0, c->stream, in, s_in, N->pk_f1, N->pk_f2,
+                       N->pk_f3, z, s_out, g.nx, g.ny, g.nz, tx, ty, zchunk, skip);
+}
+
 // backward from dz (BF16 [b][v]) and the stored forward activations: grad = dL/dW
 static void backward(hpccg_ctx *c, const float *in, const float *s_in, const Grid3 &g, int batch)
 {
@@ -950,7 +965,7 @@ int hpccg_cnn_apply_dev(hpccg_ctx *c, int32_t nx, int32_t ny, int32_t nz, const 
     const float sc[2] = {inv_scale, 1.0f / inv_scale};
     HPCCG_HIP_TRY(hipMemcpyAsync(N->scale_in, &sc[0], 4, hipMemcpyHostToDevice, c->stream));
     HPCCG_HIP_TRY(hipMemcpyAsync(N->scale_out, &sc[1], 4, hipMemcpyHostToDevice, c->stream));
-    forward(c, r, N->scale_in, z, N->scale_out, g, 1);
+    forward_fused(c, r, N->scale_in, z, N->scale_out, g);
     HPCCG_HIP_TRY(hipGetLastError());
     HPCCG_HIP_TRY(hipStreamSynchronize(c->stream));
     return 0;
@@ -1040,7 +1055,7 @@ int hpccg_fcg_learned_dev(hpccg_ctx *c, int32_t nx, int32_t ny, int32_t nz, cons
                            hrec, c->partials);
         hipLaunchKernelGGL((kf_true<kTileRows, false>), dim3(Gs), dim3(kTileRows), 0, c->stream, c->A.rowindex,
                            c->A.colindex, c->A.values16, x, b, F->r, n, nt, F->st, htrue, c->partials);
-        forward(c, F->r, s_in, F->z, s_out, g, 1, skip);
+        forward_fused(c, F->r, s_in, F->z, s_out, g, skip);
         hipLaunchKernelGGL(kf_dots<false>, dim3(Gv), dim3(kVecThreads), 0, c->stream, F->z, F->Ap, F->r, n, F->st,
                            c->partials);
         hipLaunchKernelGGL(kf_update_p<false>, dim3(Gv), dim3(kVecThreads), 0, c->stream, F->z, F->p, n, F->st);
@@ -1050,7 +1065,7 @@ int hpccg_fcg_learned_dev(hpccg_ctx *c, int32_t nx, int32_t ny, int32_t nz, cons
     // Prepare: r = b - A x, z = M(r), rz = z.r, p = z
     hipLaunchKernelGGL((kf_true<kTileRows, true>), dim3(Gs), dim3(kTileRows), 0, c->stream, c->A.rowindex,
                        c->A.colindex, c->A.values16, x, b, F->r, n, nt, F->st, htrue, c->partials);
-    forward(c, F->r, s_in, F->z, s_out, g, 1, skip);
+    forward_fused(c, F->r, s_in, F->z, s_out, g, skip);
     hipLaunchKernelGGL(kf_dots<true>, dim3(Gv), dim3(kVecThreads), 0, c->stream, F->z, F->Ap, F->r, n, F->st,
                        c->partials);
     hipLaunchKernelGGL(kf_update_p<true>, dim3(Gv), dim3(kVecThreads), 0, c->stream, F->z, F->p, n, F->st);

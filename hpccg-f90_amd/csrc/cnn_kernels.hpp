@@ -286,5 +286,198 @@ __global__ __launch_bounds__(256) void k_wgrad(const uint16_t *__restrict__ dout
     }
 }
 
+// ---- fused inference: the three layers in one kernel -------------------------------------------
+// A workgroup owns a 16 x 16 (x, y) column of output voxels and a range of z-planes, and marches
+// along z keeping three planes of each layer's input in LDS rings: step i loads input plane i
+// (rounded to BF16), computes h1 plane i-1, h2 plane i-2 and the output plane i-3.  h1 is
+// computed on a 20 x 20 and h2 on an 18 x 18 footprint (the halos the next layer needs), so the
+// activations never leave LDS.  The MFMA operands, their K order and the epilogues are those of
+// k_conv, so the output is bitwise the three-kernel forward pass's.
+constexpr int FX = 16, FY = 16;             // output footprint per workgroup
+constexpr int F2X = FX + 2, F2Y = FY + 2;   // h2 footprint (324 voxels, 21 groups of 16)
+constexpr int F1X = FX + 4, F1Y = FY + 4;   // h1 footprint (400 voxels, 25 groups)
+constexpr int F0X = FX + 6, F0Y = FY + 6;   // input footprint (484 voxels)
+constexpr int F2V = F2X * F2Y, F1V = F1X * F1Y, F0V = F0X * F0Y;
+constexpr int F2G = (F2V + 15) / 16, F1G = F1V / 16, F3G = FX * FY / 16;
+
+__device__ inline int ring3(int p) { return ((p % 3) + 3) % 3; }
+
+// Layer 1 reads its 8 taps per lane through offsets fixed per lane (the input window keeps the
+// planes in order); layers 2 and 3 resolve both candidate taps of each K-step at compile time
+// (a lane's row of the B operand picks one), load all 14 operands of a voxel group before its
+// MFMA chain, and keep h1/h2 in three-plane rings.  Two workgroups fit a CU (72 KB of LDS each).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_fused_fwd(const float *__restrict__ in, const float *__restrict__ in_scale,
+                                               const bf16x8 *__restrict__ pk1, const bf16x8 *__restrict__ pk2,
+                                               const bf16x8 *__restrict__ pk3, float *__restrict__ out,
+                                               const float *__restrict__ out_scale, int nx, int ny, int nz,
+                                               int tiles_x, int tiles_y, int zchunk, const int *__restrict__ skip)
+{
+    if (skip && *skip) return;
+    __shared__ __attribute__((aligned(16))) uint16_t U[3 * F0V];  // input window: planes i-2, i-1, i
+    __shared__ __attribute__((aligned(16))) uint16_t H1[3][F1V * 16];
+    __shared__ __attribute__((aligned(16))) uint16_t H2[3][F2V * 16];
+    const int txy = blockIdx.x % (tiles_x * tiles_y), zc = blockIdx.x / (tiles_x * tiles_y);
+    const int ox = (txy % tiles_x) * FX, oy = (txy / tiles_x) * FY;
+    const int z0 = zc * zchunk, z1 = min(z0 + zchunk, nz);
+    const float scale = in_scale ? in_scale[0] : 1.0f;
+    const float oscale = out_scale ? out_scale[0] : 1.0f;
+    const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+    const int c = lane / 16, n = lane % 16;
+    const bool hi = c >= 2;       // second tap of each K-step pair
+    const int half = (c & 1) * 8; // channel half
+    bf16x8 A1 = pk1[lane];
+    bf16x8 A2[NSTEP16], A3[NSTEP16];
+#pragma unroll
+    for (int s = 0; s < NSTEP16; ++s) { A2[s] = pk2[s * 64 + lane]; A3[s] = pk3[s * 64 + lane]; }
+    const int64_t plane = (int64_t)nx * ny;
+    int toff[8];
+    unsigned tmask = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const int t = 8 * c + j;
+        toff[j] = t < 27 ? (tap_dz(t) + 1) * F0V + (tap_dy(t) + 1) * F0X + tap_dx(t) + 1 : 0;
+        tmask |= (t < 27 ? 1u : 0u) << j;
+    }
+    uint16_t pre[2];
+    {
+        const int ip = z0 - 3;
+        const bool zin = ip >= 0 && ip < nz;
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const int v = threadIdx.x + 256 * e;
+            const int gx = ox - 3 + v % F0X, gy = oy - 3 + v / F0X;
+            pre[e] = (v < F0V && zin && gx >= 0 && gx < nx && gy >= 0 && gy < ny)
+                         ? f2bf(in[ip * plane + (int64_t)gy * nx + gx] * scale) : (uint16_t)0;
+        }
+    }
+    int r0 = ring3(z0 - 3);  // ring slot of plane z0-3; slots advance by one per step
+    for (int i = z0 - 3; i <= z1 + 2; ++i) {
+        // three ring slots: plane i -> si, i-1 -> sm1, i-2 -> sm2 (and i-3 -> si, i-4 -> sm1)
+        const int si = r0, sm1 = r0 == 0 ? 2 : r0 - 1, sm2 = sm1 == 0 ? 2 : sm1 - 1;
+        r0 = r0 == 2 ? 0 : r0 + 1;
+        {
+            // shift the window (each thread owns the same elements of all three planes) and
+            // store plane i, prefetched during the previous step
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                const int v = threadIdx.x + 256 * e;
+                if (v < F0V) {
+                    const uint16_t a1 = U[F0V + v], a2 = U[2 * F0V + v];
+                    U[v] = a1;
+                    U[F0V + v] = a2;
+                    U[2 * F0V + v] = pre[e];
+                }
+            }
+            // prefetch plane i + 1
+            const int ip = i + 1;
+            const bool zin = ip >= 0 && ip < nz;
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                const int v = threadIdx.x + 256 * e;
+                const int gx = ox - 3 + v % F0X, gy = oy - 3 + v / F0X;
+                pre[e] = (v < F0V && zin && gx >= 0 && gx < nx && gy >= 0 && gy < ny)
+                             ? f2bf(in[ip * plane + (int64_t)gy * nx + gx] * scale) : (uint16_t)0;
+            }
+        }
+        __syncthreads();
+        const int q = i - 1;  // h1 plane q uses input planes q-1, q, q+1 = window planes 0, 1, 2
+        if (q >= z0 - 2) {
+            uint16_t *Hq = H1[sm1];
+            const bool zin = q >= 0 && q < nz;
+            for (int g = w; g < F1G; g += 4) {
+                const int v = 16 * g + n, hx = v % F1X, hy = v / F1X;
+                const int gb = hy * F0X + hx;
+                uint16_t b[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const uint16_t val = U[gb + toff[j]];
+                    b[j] = (tmask >> j) & 1 ? val : (uint16_t)0;
+                }
+                u32x4 raw;
+                raw[0] = b[0] | ((uint32_t)b[1] << 16); raw[1] = b[2] | ((uint32_t)b[3] << 16);
+                raw[2] = b[4] | ((uint32_t)b[5] << 16); raw[3] = b[6] | ((uint32_t)b[7] << 16);
+                const f32x4 acc = mfma16(A1, __builtin_bit_cast(bf16x8, raw), f32x4{0.f, 0.f, 0.f, 0.f});
+                const int gx = ox - 2 + hx, gy = oy - 2 + hy;
+                const bool in_grid = zin && gx >= 0 && gx < nx && gy >= 0 && gy < ny;
+                u32x2 pk; pk[0] = pk[1] = 0u;
+                if (in_grid) {
+                    pk[0] = f2bf(acc[0] > 0.f ? acc[0] : 0.f) | ((uint32_t)f2bf(acc[1] > 0.f ? acc[1] : 0.f) << 16);
+                    pk[1] = f2bf(acc[2] > 0.f ? acc[2] : 0.f) | ((uint32_t)f2bf(acc[3] > 0.f ? acc[3] : 0.f) << 16);
+                }
+                *(u32x2 *)(Hq + v * 16 + 4 * c) = pk;
+            }
+        }
+        __syncthreads();
+        const int p = i - 2;  // h2 plane p uses H1 planes p-1 (slot of i-3 == si), p (sm2), p+1 (sm1)
+        if (p >= z0 - 1) {
+            const uint16_t *Hp3[3] = {H1[si], H1[sm2], H1[sm1]};
+            uint16_t *Hp = H2[sm2];
+            const bool zin = p >= 0 && p < nz;
+            for (int g = w; g < F2G; g += 4) {
+                const int v = 16 * g + n, vc = min(v, F2V - 1), hx = vc % F2X, hy = vc / F2X;
+                const int gb = (hy * F1X + hx) * 16 + half;
+                f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+                u32x4 raw[NSTEP16];
+#pragma unroll
+                for (int s = 0; s < NSTEP16; ++s) {
+                    const int tA = 2 * s, tB = 2 * s + 1;
+                    const uint16_t *pa = Hp3[tap_dz(tA) + 1] + ((tap_dy(tA) + 1) * F1X + tap_dx(tA) + 1) * 16;
+                    if (tB < 27) {
+                        const uint16_t *pb = Hp3[tap_dz(tB) + 1] + ((tap_dy(tB) + 1) * F1X + tap_dx(tB) + 1) * 16;
+                        raw[s] = *(const u32x4 *)((hi ? pb : pa) + gb);
+                    } else {
+                        raw[s] = *(const u32x4 *)(pa + gb);
+                        if (hi) raw[s] = u32x4{0u, 0u, 0u, 0u};
+                    }
+                }
+#pragma unroll
+                for (int s = 0; s < NSTEP16; ++s) acc = mfma16(A2[s], __builtin_bit_cast(bf16x8, raw[s]), acc);
+                // schedule: the 14 LDS reads first, then the 14 MFMAs (the default schedule
+                // issues each read just before its MFMA and waits on it)
+                __builtin_amdgcn_sched_group_barrier(0x100, 14, 0);
+                __builtin_amdgcn_sched_group_barrier(0x008, 14, 0);
+                const int gx = ox - 1 + hx, gy = oy - 1 + hy;
+                const bool in_grid = zin && gx >= 0 && gx < nx && gy >= 0 && gy < ny;
+                u32x2 pk; pk[0] = pk[1] = 0u;
+                if (in_grid) {
+                    pk[0] = f2bf(acc[0] > 0.f ? acc[0] : 0.f) | ((uint32_t)f2bf(acc[1] > 0.f ? acc[1] : 0.f) << 16);
+                    pk[1] = f2bf(acc[2] > 0.f ? acc[2] : 0.f) | ((uint32_t)f2bf(acc[3] > 0.f ? acc[3] : 0.f) << 16);
+                }
+                if (v < F2V) *(u32x2 *)(Hp + v * 16 + 4 * c) = pk;
+            }
+        }
+        __syncthreads();
+        const int o = i - 3;  // output plane o uses H2 planes o-1 (slot of i-4 == sm1), o (si), o+1 (sm2)
+        if (o >= z0 && o < z1) {
+            const uint16_t *Ho3[3] = {H2[sm1], H2[si], H2[sm2]};
+            for (int g = w; g < F3G; g += 4) {
+                const int gb = (g * F2X + n) * 16 + half;
+                f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+                u32x4 raw[NSTEP16];
+#pragma unroll
+                for (int s = 0; s < NSTEP16; ++s) {
+                    const int tA = 2 * s, tB = 2 * s + 1;
+                    const uint16_t *pa = Ho3[tap_dz(tA) + 1] + ((tap_dy(tA) + 1) * F2X + tap_dx(tA) + 1) * 16;
+                    if (tB < 27) {
+                        const uint16_t *pb = Ho3[tap_dz(tB) + 1] + ((tap_dy(tB) + 1) * F2X + tap_dx(tB) + 1) * 16;
+                        raw[s] = *(const u32x4 *)((hi ? pb : pa) + gb);
+                    } else {
+                        raw[s] = *(const u32x4 *)(pa + gb);
+                        if (hi) raw[s] = u32x4{0u, 0u, 0u, 0u};
+                    }
+                }
+#pragma unroll
+                for (int s = 0; s < NSTEP16; ++s) acc = mfma16(A3[s], __builtin_bit_cast(bf16x8, raw[s]), acc);
+                // schedule: the 14 LDS reads first, then the 14 MFMAs (the default schedule
+                // issues each read just before its MFMA and waits on it)
+                __builtin_amdgcn_sched_group_barrier(0x100, 14, 0);
+                __builtin_amdgcn_sched_group_barrier(0x008, 14, 0);
+                const int gx = ox + n, gy = oy + g;
+                if (c == 0 && gx < nx && gy < ny) out[o * plane + (int64_t)gy * nx + gx] = acc[0] * oscale;
+            }
+        }
+    }
+}
+
 }  // namespace cnn
 }  // namespace hpccg

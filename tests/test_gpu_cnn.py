@@ -74,6 +74,29 @@ def test_mfma_operand_layout_single_tap():
         c.close()
 
 
+@pytest.mark.parametrize("dims", [(16, 16, 16), (37, 21, 19), (5, 40, 3), (48, 32, 70)])
+def test_fused_inference_equals_layerwise_forward(ctx, dims):
+    """The fused three-layer inference kernel (activations kept in LDS while a workgroup marches
+    along z) uses the per-layer kernels' MFMA operands and K order, so its output is bitwise
+    theirs -- on grids that are not multiples of its 16 x 16 footprint, with few z-planes, and
+    with more z-chunks than one; and it honours the input/output scales."""
+    w = he_weights(7)
+    ctx.cnn_set_weights(w)
+    n = int(np.prod(dims))
+    u = np.random.default_rng(8).standard_normal(n).astype(np.float32)
+    z_layers = ctx.cnn_forward(u, dims, 1)
+    ud = hpccg_amd.DeviceArray.from_numpy(ctx, u)
+    zd = hpccg_amd.DeviceArray(ctx, n, np.float32)
+    ctx.cnn_apply_dev(dims, ud, 1.0, zd)
+    assert np.array_equal(zd.numpy(), z_layers)
+    # z = s * net(u / s): the network is positively homogeneous, so only BF16 rounding differs
+    s = 4.0  # a power of two: the scaled input rounds exactly like the unscaled one
+    ctx.cnn_apply_dev(dims, ud, 1.0 / s, zd)
+    assert np.array_equal(zd.numpy(), z_layers)
+    zo, _, _ = oracle.cnn_forward(w, u, dims, 1, quant=True)
+    assert np.linalg.norm(zd.numpy() - zo) <= 2e-2 * np.linalg.norm(zo)
+
+
 @pytest.mark.parametrize("dims,batch", [((8, 8, 4), 1), ((11, 6, 9), 2)])
 def test_backward_matches_oracle(ctx, dims, batch):
     w = he_weights(4)
