@@ -50,18 +50,18 @@ struct CnnNet {
 static CnnNet *net_of(hpccg_ctx *c) { return (CnnNet *)c->ext[1]; }
 
 // ---- weight packing into MFMA A-operands -------------------------------------------------------
-// pack_f1[l][j] = W1[co=l%16][t=8(l/16)+j]; pack_f2[s][l][j] = W2[co][ci=8((l/16)&1)+j][t=2s+(l/16)/2];
-// pack_f3 as pack_f2 with co = 0 only (W3[ci][t]); pack_b3[l][j] = W3[l%16][26 - t];
+// pack_f1[l][j] = W1[co=l%16][t=k1_tap(l/16, j)]; pack_f2[s][l][j] = W2[co][ci=8((l/16)&1)+j][t=2s+(l/16)/2];
+// pack_f3 as pack_f2 with co = 0 only (W3[ci][t]); pack_b3[l][j] = W3[l%16][26 - k1_tap(l/16, j)];
 // pack_b2[s][l][j] = W2[ci'][co'=l%16][26 - t] with ci' = 8((l/16)&1)+j (transposed, flipped).
 __global__ void k_pack(const float *__restrict__ W, uint16_t *f1, uint16_t *f2, uint16_t *f3, uint16_t *b3,
                        uint16_t *b2)
 {
     const float *W1 = W, *W2 = W + NW1, *W3 = W + NW1 + NW2;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < 64 * 8) {  // 1-step packs
-        const int l = i / 8, j = i % 8, co = l % 16, t = 8 * (l / 16) + j;
-        f1[i] = t < 27 ? f2bf(W1[co * 27 + t]) : 0;
-        b3[i] = t < 27 ? f2bf(W3[co * 27 + (26 - t)]) : 0;
+    if (i < 64 * 8) {  // 1-step packs (one-channel input: K order k1_tap)
+        const int l = i / 8, j = i % 8, co = l % 16, t = k1_tap(l / 16, j);
+        f1[i] = t >= 0 ? f2bf(W1[co * 27 + t]) : 0;
+        b3[i] = t >= 0 ? f2bf(W3[co * 27 + (26 - t)]) : 0;
     }
     if (i < NSTEP16 * 64 * 8) {
         const int s = i / 512, l = (i / 8) % 64, j = i % 8;

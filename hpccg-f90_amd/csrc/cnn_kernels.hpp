@@ -60,6 +60,18 @@ __device__ inline int tap_dx(int t) { return t % 3 - 1; }
 __device__ inline int tap_dy(int t) { return (t / 3) % 3 - 1; }
 __device__ inline int tap_dz(int t) { return t / 9 - 1; }
 
+// K order of a one-channel input (27 taps in K = 32): lane row c holds the x-triplets of tap
+// lines 2c and 2c+1 in slots 0-5 (tap 6c + j), and the last line (taps 24-26) fills slots 6-7
+// of rows 0 and 1; -1 marks a zero slot.  A line is (dz, dy) with dx = -1, 0, 1 consecutive, so a
+// lane reads whole triplets.
+__host__ __device__ inline int k1_tap(int c, int j)
+{
+    if (j < 6) return 6 * c + j;
+    if (c == 0) return 24 + (j - 6);
+    if (c == 1 && j == 6) return 26;
+    return -1;
+}
+
 __device__ inline f32x4 mfma16(bf16x8 a, bf16x8 b, f32x4 c)
 {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
@@ -163,9 +175,9 @@ __global__ __launch_bounds__(256) void k_conv(const void *__restrict__ in, const
             uint16_t v[8];
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
-                const int t = 8 * c + j;
+                const int t = k1_tap(c, j);
                 v[j] = 0;
-                if (t < 27) {
+                if (t >= 0) {
                     const int hv = (z + tap_dz(t) + 1) * (HY * HX) + (y + tap_dy(t) + 1) * HX + (x + tap_dx(t) + 1);
                     v[j] = sm[hv];
                 }
@@ -302,10 +314,10 @@ constexpr int F2G = (F2V + 15) / 16, F1G = F1V / 16, F3G = FX * FY / 16;
 
 __device__ inline int ring3(int p) { return ((p % 3) + 3) % 3; }
 
-// Layer 1 reads its 8 taps per lane through offsets fixed per lane (the input window keeps the
-// planes in order); layers 2 and 3 resolve both candidate taps of each K-step at compile time
+// Layer 1 reads its taps as three x-triplets per lane (the input window keeps its planes in
+// order, so the offsets are fixed per lane); layers 2 and 3 resolve both candidate taps of each K-step at compile time
 // (a lane's row of the B operand picks one), load all 14 operands of a voxel group before its
-// MFMA chain, and keep h1/h2 in three-plane rings.  Two workgroups fit a CU (72 KB of LDS each).
+// MFMA chain, and keep h1/h2 in three-plane rings.  Two workgroups fit a CU (78 KB of LDS each).
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_fused_fwd(const float *__restrict__ in, const float *__restrict__ in_scale,
                                                const bf16x8 *__restrict__ pk1, const bf16x8 *__restrict__ pk2,
                                                const bf16x8 *__restrict__ pk3, float *__restrict__ out,
@@ -313,7 +325,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
                                                int tiles_x, int tiles_y, int zchunk, const int *__restrict__ skip)
 {
     if (skip && *skip) return;
-    __shared__ __attribute__((aligned(16))) uint16_t U[3 * F0V];  // input window: planes i-2, i-1, i
+    // input window, planes i-2, i-1, i, stored as x-triplets: T[plane][row][x] holds the input at
+    // window columns x, x+1, x+2 (one 8-byte read per tap line of layer 1)
+    __shared__ __attribute__((aligned(16))) u32x2 T[3][F0Y][F1X];
     __shared__ __attribute__((aligned(16))) uint16_t H1[3][F1V * 16];
     __shared__ __attribute__((aligned(16))) uint16_t H2[3][F2V * 16];
     const int txy = blockIdx.x % (tiles_x * tiles_y), zc = blockIdx.x / (tiles_x * tiles_y);
@@ -330,54 +344,53 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
 #pragma unroll
     for (int s = 0; s < NSTEP16; ++s) { A2[s] = pk2[s * 64 + lane]; A3[s] = pk3[s * 64 + lane]; }
     const int64_t plane = (int64_t)nx * ny;
-    int toff[8];
-    unsigned tmask = 0;
+    // layer-1 tap lines of this lane's B-operand row (k1_tap order): lines 2c, 2c+1 and line 8
+    const int lA = 2 * c, lB = 2 * c + 1;
+    const int offA = ((lA / 3) * F0Y + lA % 3) * F1X, offB = ((lB / 3) * F0Y + lB % 3) * F1X;
+    const int offC = (2 * F0Y + 2) * F1X;
+    const u32x2 *T0 = &T[0][0][0];
+    // triplet e of a plane: window row e / F1X, columns e % F1X .. +2 (440 per plane, two per thread)
+    auto load_triplet = [&](int ip, int e) -> u32x2 {
+        u32x2 t3;
+        t3[0] = t3[1] = 0u;
+        if (e >= F0Y * F1X || ip < 0 || ip >= nz) return t3;
+        const int gy = oy - 3 + e / F1X, gx0 = ox - 3 + e % F1X;
+        if (gy < 0 || gy >= ny) return t3;
+        const float *row = in + ip * plane + (int64_t)gy * nx;
+        uint16_t u[3];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        const int t = 8 * c + j;
-        toff[j] = t < 27 ? (tap_dz(t) + 1) * F0V + (tap_dy(t) + 1) * F0X + tap_dx(t) + 1 : 0;
-        tmask |= (t < 27 ? 1u : 0u) << j;
-    }
-    uint16_t pre[2];
-    {
-        const int ip = z0 - 3;
-        const bool zin = ip >= 0 && ip < nz;
-#pragma unroll
-        for (int e = 0; e < 2; ++e) {
-            const int v = threadIdx.x + 256 * e;
-            const int gx = ox - 3 + v % F0X, gy = oy - 3 + v / F0X;
-            pre[e] = (v < F0V && zin && gx >= 0 && gx < nx && gy >= 0 && gy < ny)
-                         ? f2bf(in[ip * plane + (int64_t)gy * nx + gx] * scale) : (uint16_t)0;
+        for (int k = 0; k < 3; ++k) {
+            const int gx = gx0 + k;
+            u[k] = (gx >= 0 && gx < nx) ? f2bf(row[gx] * scale) : (uint16_t)0;
         }
-    }
+        t3[0] = u[0] | ((uint32_t)u[1] << 16);
+        t3[1] = u[2];
+        return t3;
+    };
+    u32x2 pre[2];
+#pragma unroll
+    for (int e = 0; e < 2; ++e) pre[e] = load_triplet(z0 - 3, threadIdx.x + 256 * e);
     int r0 = ring3(z0 - 3);  // ring slot of plane z0-3; slots advance by one per step
     for (int i = z0 - 3; i <= z1 + 2; ++i) {
         // three ring slots: plane i -> si, i-1 -> sm1, i-2 -> sm2 (and i-3 -> si, i-4 -> sm1)
         const int si = r0, sm1 = r0 == 0 ? 2 : r0 - 1, sm2 = sm1 == 0 ? 2 : sm1 - 1;
         r0 = r0 == 2 ? 0 : r0 + 1;
         {
-            // shift the window (each thread owns the same elements of all three planes) and
-            // store plane i, prefetched during the previous step
+            // shift the window (each thread owns the same triplets of all three planes) and
+            // store plane i, prefetched during the previous step; then prefetch plane i + 1
 #pragma unroll
             for (int e = 0; e < 2; ++e) {
                 const int v = threadIdx.x + 256 * e;
-                if (v < F0V) {
-                    const uint16_t a1 = U[F0V + v], a2 = U[2 * F0V + v];
-                    U[v] = a1;
-                    U[F0V + v] = a2;
-                    U[2 * F0V + v] = pre[e];
+                if (v < F0Y * F1X) {
+                    u32x2 *Tv = &T[0][0][0] + v;
+                    const u32x2 a1 = Tv[F0Y * F1X], a2 = Tv[2 * F0Y * F1X];
+                    Tv[0] = a1;
+                    Tv[F0Y * F1X] = a2;
+                    Tv[2 * F0Y * F1X] = pre[e];
                 }
             }
-            // prefetch plane i + 1
-            const int ip = i + 1;
-            const bool zin = ip >= 0 && ip < nz;
 #pragma unroll
-            for (int e = 0; e < 2; ++e) {
-                const int v = threadIdx.x + 256 * e;
-                const int gx = ox - 3 + v % F0X, gy = oy - 3 + v / F0X;
-                pre[e] = (v < F0V && zin && gx >= 0 && gx < nx && gy >= 0 && gy < ny)
-                             ? f2bf(in[ip * plane + (int64_t)gy * nx + gx] * scale) : (uint16_t)0;
-            }
+            for (int e = 0; e < 2; ++e) pre[e] = load_triplet(i + 1, threadIdx.x + 256 * e);
         }
         __syncthreads();
         const int q = i - 1;  // h1 plane q uses input planes q-1, q, q+1 = window planes 0, 1, 2
@@ -386,16 +399,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
             const bool zin = q >= 0 && q < nz;
             for (int g = w; g < F1G; g += 4) {
                 const int v = 16 * g + n, hx = v % F1X, hy = v / F1X;
-                const int gb = hy * F0X + hx;
-                uint16_t b[8];
-#pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    const uint16_t val = U[gb + toff[j]];
-                    b[j] = (tmask >> j) & 1 ? val : (uint16_t)0;
-                }
+                const int gb = hy * F1X + hx;
+                const u32x2 ta = T0[offA + gb], tb = T0[offB + gb], tc = T0[offC + gb];
                 u32x4 raw;
-                raw[0] = b[0] | ((uint32_t)b[1] << 16); raw[1] = b[2] | ((uint32_t)b[3] << 16);
-                raw[2] = b[4] | ((uint32_t)b[5] << 16); raw[3] = b[6] | ((uint32_t)b[7] << 16);
+                raw[0] = ta[0];
+                raw[1] = (ta[1] & 0xffffu) | (tb[0] << 16);
+                raw[2] = (tb[0] >> 16) | (tb[1] << 16);
+                raw[3] = c == 0 ? tc[0] : (c == 1 ? (tc[1] & 0xffffu) : 0u);
                 const f32x4 acc = mfma16(A1, __builtin_bit_cast(bf16x8, raw), f32x4{0.f, 0.f, 0.f, 0.f});
                 const int gx = ox - 2 + hx, gy = oy - 2 + hy;
                 const bool in_grid = zin && gx >= 0 && gx < nx && gy >= 0 && gy < ny;
