@@ -324,6 +324,26 @@ __global__ __launch_bounds__(kVecThreads) void k_dot_tree(const double *__restri
         *out = tot;
 }
 
+// max_i |v1_i - v2_i| (numerics.F90:178-183): per-thread maxima, a wavefront/workgroup max, and
+// one atomic max per workgroup on the bit pattern of the non-negative result (bit patterns of
+// non-negative doubles order like the values), so the result is exact and order-free
+__global__ __launch_bounds__(kVecThreads) void k_maxabs_diff(const double *__restrict__ v1,
+                                                             const double *__restrict__ v2, int n,
+                                                             unsigned long long *out)
+{
+    __shared__ double red[kVecThreads / kWave];
+    double m = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)kVecThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kVecThreads)
+        m = fmax(m, fabs(v1[i] - v2[i]));
+    for (int off = kWave / 2; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, kWave));
+    if (threadIdx.x % kWave == 0) red[threadIdx.x / kWave] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kVecThreads / kWave; ++w) m = fmax(m, red[w]);
+        atomicMax(out, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
 // w = alpha*x + beta*y      vector_operations.F90:116-119
 __global__ void k_waxpby(int n, double alpha, const double *__restrict__ x, double beta,
                          const double *__restrict__ y, double *__restrict__ w)
@@ -990,6 +1010,24 @@ int hpccg_ddot_dev(hpccg_ctx *c, int32_t n, const double *x, const double *y, do
 
 static int vec_launch_grid(hpccg_ctx *c, int n) { return std::max(1, std::min((n + 255) / 256, c->info.num_cu * 8)); }
 
+int hpccg_compute_residual_dev(hpccg_ctx *c, int32_t n, const double *v1, const double *v2, double *residual)
+{
+    HPCCG_CHECK_ARG(c && residual && n >= 0 && (n == 0 || (v1 && v2)), "bad argument");
+    if (n == 0) {
+        *residual = 0.0;
+        return 0;
+    }
+    HPCCG_HIP_TRY(hipSetDevice(c->device));
+    HPCCG_HIP_TRY(hipMemsetAsync(c->scalar, 0, sizeof(double), c->stream));
+    hipLaunchKernelGGL(k_maxabs_diff, dim3(vec_launch_grid(c, n)), dim3(kVecThreads), 0, c->stream, v1, v2, n,
+                       (unsigned long long *)c->scalar);
+    HPCCG_HIP_TRY(hipGetLastError());
+    HPCCG_HIP_TRY(hipMemcpyAsync(c->scalar_h, c->scalar, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HPCCG_HIP_TRY(hipStreamSynchronize(c->stream));
+    *residual = *c->scalar_h;
+    return 0;
+}
+
 int hpccg_waxpby_dev(hpccg_ctx *c, int32_t n, double alpha, const double *x, double beta, const double *y, double *w)
 {
     HPCCG_CHECK_ARG(c && n >= 0 && (n == 0 || (x && y && w)), "bad argument");
@@ -1065,6 +1103,19 @@ int hpccg_ddot1(hpccg_ctx *c, int32_t n, const double *x, double *r)
     HPCCG_TRY(ensure_stage(c, n));
     HPCCG_HIP_TRY(hipMemcpyAsync(c->s0, x, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
     return hpccg_ddot_dev(c, n, c->s0, c->s0, r);
+}
+
+int hpccg_compute_residual(hpccg_ctx *c, int32_t n, const double *v1, const double *v2, double *residual)
+{
+    HPCCG_CHECK_ARG(c && residual && n >= 0 && (n == 0 || (v1 && v2)), "bad argument");
+    if (n == 0) {
+        *residual = 0.0;
+        return 0;
+    }
+    HPCCG_TRY(ensure_stage(c, n));
+    HPCCG_HIP_TRY(hipMemcpyAsync(c->s0, v1, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HPCCG_HIP_TRY(hipMemcpyAsync(c->s1, v2, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    return hpccg_compute_residual_dev(c, n, c->s0, c->s1, residual);
 }
 
 int hpccg_waxpby(hpccg_ctx *c, int32_t n, double alpha, const double *x, double beta, const double *y, double *w)

@@ -19,6 +19,7 @@ module hpccg_capi
     public :: hpccg_context, hpccg_check, hpccg_new_tag, hpccg_resident_tag, hpccg_set_resident
     public :: c_hpccg_matrix_upload, c_hpccg_generate_matrix, c_hpccg_spmv, c_hpccg_ddot, c_hpccg_ddot1
     public :: c_hpccg_waxpby, c_hpccg_wwpby, c_hpccg_wxbw, c_hpccg_vcopy, c_hpccg_cg
+    public :: c_hpccg_compute_residual
 
     interface
         integer(c_int) function c_hpccg_create(ctx, device) bind(C, name='hpccg_create')
@@ -69,6 +70,13 @@ module hpccg_capi
             type(c_ptr), value :: ctx
             integer(c_int32_t), value :: n
             real(c_double), intent(in) :: x(*)
+            real(c_double) :: r
+        end function
+        integer(c_int) function c_hpccg_compute_residual(ctx, n, v1, v2, r) bind(C, name='hpccg_compute_residual')
+            import :: c_ptr, c_int, c_int32_t, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: n
+            real(c_double), intent(in) :: v1(*), v2(*)
             real(c_double) :: r
         end function
         integer(c_int) function c_hpccg_waxpby(ctx, n, alpha, x, beta, y, w) bind(C, name='hpccg_waxpby')

@@ -53,16 +53,18 @@ contains
     end subroutine cg
 
     ! max_i |v1(i) - v2(i)|  (numerics.F90:164-198; single rank, no MAX allreduce)
+    ! max |v1 - v2| on the GPU (numerics.F90:164-198); every call starts from 0 (the reference's
+    ! local_residual is implicitly SAVEd, so only its first call is meaningful)
     subroutine compute_residual(n, v1, v2, residual)
+        use iso_c_binding
+        use hpccg_capi
         implicit none
         integer, intent(in) :: n
         real(kind=dp), allocatable, intent(inout) :: v1(:)
         real(kind=dp), allocatable, intent(inout) :: v2(:)
         real(kind=dp), intent(out) :: residual
-        integer :: i
-        residual = 0.0_dp
-        do i = 1, n
-            residual = max(residual, abs(v1(i) - v2(i)))
-        end do
+        real(c_double) :: r
+        call hpccg_check(c_hpccg_compute_residual(hpccg_context(), int(n, c_int32_t), v1, v2, r), 'compute_residual')
+        residual = r
     end subroutine compute_residual
 end module numerics

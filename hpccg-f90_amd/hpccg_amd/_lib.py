@@ -42,6 +42,8 @@ _SIGS = {
     "hpccg_spmv": (ctypes.c_int, [P, P, P]),
     "hpccg_ddot": (ctypes.c_int, [P, I32, P, P, PD]),
     "hpccg_ddot1": (ctypes.c_int, [P, I32, P, PD]),
+    "hpccg_compute_residual": (ctypes.c_int, [P, I32, P, P, PD]),
+    "hpccg_compute_residual_dev": (ctypes.c_int, [P, I32, P, P, PD]),
     "hpccg_waxpby": (ctypes.c_int, [P, I32, D, P, D, P, P]),
     "hpccg_wwpby": (ctypes.c_int, [P, I32, D, P, P]),
     "hpccg_wxbw": (ctypes.c_int, [P, I32, P, D, P]),
@@ -267,6 +269,18 @@ class Context:
         x = np.ascontiguousarray(x, dtype=np.float64)
         r = ctypes.c_double()
         _check(lib().hpccg_ddot1(self.h, len(x), _ptr(x), ctypes.byref(r)), "ddot1")
+        return r.value
+
+    def compute_residual(self, v1, v2):
+        v1 = np.ascontiguousarray(v1, dtype=np.float64)
+        v2 = np.ascontiguousarray(v2, dtype=np.float64)
+        r = ctypes.c_double()
+        _check(lib().hpccg_compute_residual(self.h, len(v1), _ptr(v1), _ptr(v2), ctypes.byref(r)), "compute_residual")
+        return r.value
+
+    def compute_residual_dev(self, v1: "DeviceArray", v2: "DeviceArray"):
+        r = ctypes.c_double()
+        _check(lib().hpccg_compute_residual_dev(self.h, v1.n, v1.ptr, v2.ptr, ctypes.byref(r)), "compute_residual_dev")
         return r.value
 
     def waxpby(self, alpha, x, beta, y):

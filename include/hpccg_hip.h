@@ -91,6 +91,13 @@ int hpccg_wxbw(hpccg_ctx *ctx, int32_t n, const double *x, double beta, double *
 /* vcopy(n, x, y): y = x                                     vector_operations.F90:155-169 */
 int hpccg_vcopy(hpccg_ctx *ctx, int32_t n, const double *x, double *y);
 
+/* compute_residual(n, v1, v2, residual): max_i |v1_i - v2_i|   numerics.F90:164-198
+ * Exact (a maximum is order-free).  Unlike the reference, whose local_residual keeps its value
+ * between calls (initialised in its declaration, numerics.F90:177, hence implicitly SAVEd), every
+ * call starts from 0; the result equals the reference's first call.  NaN differences are skipped. */
+int hpccg_compute_residual(hpccg_ctx *ctx, int32_t n, const double *v1, const double *v2,
+                           double *residual);
+
 /* cg(A, b, x, max_iter, tolerance, niters, normr)           numerics.F90:41-162
  * Unpreconditioned CG from the x passed in (updated in place), same operation order, same
  * lagging convergence test (exit at the top of iteration k when ||r_{k-2}|| < tolerance),
@@ -109,6 +116,8 @@ int hpccg_waxpby_dev(hpccg_ctx *ctx, int32_t n, double alpha, const double *x, d
                      const double *y, double *w);
 int hpccg_wwpby_dev(hpccg_ctx *ctx, int32_t n, double beta, const double *y, double *w);
 int hpccg_wxbw_dev(hpccg_ctx *ctx, int32_t n, const double *x, double beta, double *w);
+int hpccg_compute_residual_dev(hpccg_ctx *ctx, int32_t n, const double *v1, const double *v2,
+                               double *residual);
 /* cg with b and x resident on the device.  *elapsed_ms (may be NULL) receives the device time
  * of the whole solve -- Prepare, iteration loop and final x update (HIP events on the
  * context's stream; the reference's "Total" timer, numerics.F90:106, covers the loop only). */

@@ -101,6 +101,25 @@ def test_dots_ragged_lengths(ctx, n):
     assert ctx.ddot1(xi) == float(np.sum(xi.astype(np.int64) ** 2))  # exact in any order
 
 
+@pytest.mark.parametrize("n", [0, 1, 63, 257, 100003, 2_000_003])
+def test_compute_residual_exact(ctx, n):
+    """max |v1 - v2| equals the oracle's bit for bit (a maximum is order-free); repeated calls do
+    not carry the previous result (the reference's SAVEd accumulator is not reproduced)."""
+    v1 = oracle.xstar_uniform(n, 31 + n)
+    v2 = oracle.xstar_uniform(n, 37 + n)
+    assert ctx.compute_residual(v1, v2) == oracle.compute_residual(v1, v2)
+    if n:
+        small = v2 + 1e-3 * (v1 - v2)
+        assert ctx.compute_residual(v1, small) == oracle.compute_residual(v1, small)
+        d1 = hpccg_amd.DeviceArray.from_numpy(ctx, v1)
+        d2 = hpccg_amd.DeviceArray.from_numpy(ctx, small)
+        assert ctx.compute_residual_dev(d1, d2) == oracle.compute_residual(v1, small)
+        spike = v1.copy()
+        spike[n // 2] += 7.5  # the maximum sits in one element
+        r = ctx.compute_residual(spike, v1)
+        assert r == oracle.compute_residual(spike, v1) and abs(r - 7.5) < 1e-14
+
+
 def test_spmv_generic_csr_long_rows(ctx):
     """A CSR that is not a 27-point stencil (rows up to 200 nnz): the per-row fallback path."""
     rng = np.random.default_rng(3)
