@@ -52,9 +52,9 @@ contains
         deallocate(hist)
     end subroutine cg
 
-    ! max_i |v1(i) - v2(i)|  (numerics.F90:164-198; single rank, no MAX allreduce)
-    ! max |v1 - v2| on the GPU (numerics.F90:164-198); every call starts from 0 (the reference's
-    ! local_residual is implicitly SAVEd, so only its first call is meaningful)
+    ! max_i |v1(i) - v2(i)| on the GPU (numerics.F90:164-198; single rank, no MAX allreduce);
+    ! every call starts from 0 (the reference's local_residual is implicitly SAVEd, so only its
+    ! first call is meaningful)
     subroutine compute_residual(n, v1, v2, residual)
         use iso_c_binding
         use hpccg_capi
