@@ -93,7 +93,7 @@ def measure_roofline(ctx, xs, n, nnz, n1, reps):
             traffic = json.load(f).get("traffic_bytes")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "k_cg_spmv<64,true> (FP64 CSR SpMV + p.Ap epilogue)",
+                "kernel": "k_cg_spmv<true> (FP64 sliced-ELL SpMV of the CSR + p.Ap partials)",
                 "algorithmic_bytes": B, "avg_ms": round(spmv_ms, 5)}
     cb = min(B // 2, 2 ** 31) // 16 * 16
     ca = hpccg_amd.DeviceArray(ctx, cb // 8)

@@ -56,6 +56,12 @@ struct Batch {
     int32_t *colindex = nullptr;        // nnz, global 1-based
     __half *values = nullptr;           // nnz (27 / -1 exactly)
     int64_t *row_off = nullptr;         // device copy of row_off_h
+    // sliced-ELL copy (spmv.hpp), slices never straddle two systems
+    int64_t *slice_ptr = nullptr;       // [total slices + 1] slot offsets
+    int32_t *scol = nullptr;            // 0-based global columns
+    __half *sval = nullptr;
+    int *slice_off = nullptr;           // [S+1] first slice of each system
+    int64_t slots = 0;
     float *r = nullptr, *p = nullptr, *Ap = nullptr;
     BatchState *st = nullptr;
     double *partials = nullptr;         // one slot per logical workgroup
@@ -129,40 +135,63 @@ __device__ __forceinline__ bool team_pos(const int *wg_team, const int *team_fir
     return true;
 }
 
+// The batch's sliced copy as the SpMV kernels take it.
+struct BatchSell {
+    const int64_t *slice_ptr;
+    const int32_t *col;
+    const __half *val;
+    const int32_t *rowptr;  // global 1-based row pointer
+    const int *slice_off;   // first slice of each system
+};
+
+static inline BatchSell batch_sell(const Batch *B) { return BatchSell{B->slice_ptr, B->scol, B->sval, B->rowindex, B->slice_off}; }
+
+// A team's workgroup walks tiles of ROWS/64 slices of its system (one slice per wave) over a
+// contiguous tile range; f(global row, row sum) runs for every row of the system it covers.
+template <int ROWS, typename XT, typename F>
+__device__ __forceinline__ void team_rows(const BatchSell &A, const TeamPos &tp, const XT *__restrict__ x, F f)
+{
+    constexpr int W = ROWS / kSlice;
+    const int nsl = (int)((tp.n + kSlice - 1) / kSlice);
+    const int ntiles = (nsl + W - 1) / W;
+    const int t0 = (int)(((int64_t)tp.l * ntiles) / tp.g), t1 = (int)(((int64_t)(tp.l + 1) * ntiles) / tp.g);
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kSlice), lane = threadIdx.x % kSlice;
+    const int soff = A.slice_off[tp.s];
+    for (int t = t0; t < t1; ++t) {
+        const int ls = t * W + w;
+        if (ls >= nsl) continue;  // wave-uniform
+        const int64_t rl = (int64_t)ls * kSlice + lane;
+        const bool valid = rl < tp.n;
+        const int64_t gr = tp.off + rl;
+        const double sum = sell_rowsum<kGroupF16, __half, XT>(A.slice_ptr, A.col, A.val, A.rowptr, x, soff + ls, lane,
+                                                             gr, valid);
+        if (valid) f(gr, sum);
+    }
+}
+
 // ---- kernels ------------------------------------------------------------------------------------
 // True residual of every system that asked for one (PREP: all systems, also forms ||b||^2).
 template <int ROWS, bool PREP>
-__global__ __launch_bounds__(ROWS) void kb_true(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
-                                                const __half *__restrict__ val, const double *__restrict__ x,
+__global__ __launch_bounds__(ROWS) void kb_true(const BatchSell A, const double *__restrict__ x,
                                                 const double *__restrict__ b, float *__restrict__ r,
                                                 const int64_t *__restrict__ row_off, const int *wg_team,
                                                 const int *team_first, const int *team_size, BatchState *st,
                                                 double *partials, double rel_tol2)
 {
-    __shared__ double lds[ROWS * 27];
     __shared__ double red[ROWS / kWave];
     TeamPos tp;
     int lb;
     if (!team_pos(wg_team, team_first, team_size, row_off, tp, lb)) return;
     BatchState *S = st + tp.s;
     if (!PREP && (!S->active || !S->need_true)) return;
-    const int ntiles = (int)((tp.n + ROWS - 1) / ROWS);
-    const int t0 = (int)(((int64_t)tp.l * ntiles) / tp.g), t1 = (int)(((int64_t)(tp.l + 1) * ntiles) / tp.g);
     double rr = 0.0, bb = 0.0;
-    for (int t = t0; t < t1; ++t) {
-        bool valid;
-        int rl;
-        const double ax = spmv_tile_rowsum<ROWS, __half, double>(rowptr + tp.off, col, val, x, (int)tp.n, t, lds,
-                                                                 valid, rl);
-        if (valid) {
-            const int64_t gr = tp.off + rl;
-            const double bi = b[gr];
-            const float ri = (float)(bi - ax);
-            r[gr] = ri;
-            rr = fma((double)ri, (double)ri, rr);
-            if (PREP) bb = fma(bi, bi, bb);
-        }
-    }
+    team_rows<ROWS>(A, tp, x, [&](int64_t gr, double ax) {
+        const double bi = b[gr];
+        const float ri = (float)(bi - ax);
+        r[gr] = ri;
+        rr = fma((double)ri, (double)ri, rr);
+        if (PREP) bb = fma(bi, bi, bb);
+    });
     rr = block_sum<ROWS>(rr, red);
     if (PREP) bb = block_sum<ROWS>(bb, red);
     __shared__ int is_last;
@@ -231,32 +260,23 @@ __global__ __launch_bounds__(ROWS) void kb_update_p(const float *__restrict__ r,
 }
 
 template <int ROWS>
-__global__ __launch_bounds__(ROWS) void kb_spmv(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
-                                                const __half *__restrict__ val, const float *__restrict__ p,
-                                                float *__restrict__ Ap, const int64_t *__restrict__ row_off,
-                                                const int *wg_team, const int *team_first, const int *team_size,
-                                                BatchState *st, double *partials)
+__global__ __launch_bounds__(ROWS) void kb_spmv(const BatchSell A, const float *__restrict__ p, float *__restrict__ Ap,
+                                                const int64_t *__restrict__ row_off, const int *wg_team,
+                                                const int *team_first, const int *team_size, BatchState *st,
+                                                double *partials)
 {
-    __shared__ double lds[ROWS * 27];
     __shared__ double red[ROWS / kWave];
     TeamPos tp;
     int lb;
     if (!team_pos(wg_team, team_first, team_size, row_off, tp, lb)) return;
     BatchState *S = st + tp.s;
     if (!S->active) return;
-    const int ntiles = (int)((tp.n + ROWS - 1) / ROWS);
-    const int t0 = (int)(((int64_t)tp.l * ntiles) / tp.g), t1 = (int)(((int64_t)(tp.l + 1) * ntiles) / tp.g);
     double dot = 0.0;
-    for (int t = t0; t < t1; ++t) {
-        bool valid;
-        int rl;
-        const double s = spmv_tile_rowsum<ROWS, __half, float>(rowptr + tp.off, col, val, p, (int)tp.n, t, lds, valid, rl);
-        if (valid) {
-            const float a = (float)s;
-            Ap[tp.off + rl] = a;
-            dot = fma((double)p[tp.off + rl], (double)a, dot);
-        }
-    }
+    team_rows<ROWS>(A, tp, p, [&](int64_t gr, double s) {
+        const float a = (float)s;
+        Ap[gr] = a;
+        dot = fma((double)p[gr], (double)a, dot);
+    });
     dot = block_sum<ROWS>(dot, red);
     double pap;
     if (grid_sum<ROWS>(dot, partials + (lb - tp.l), &S->counter[1], tp.g, tp.l, red, &pap) && threadIdx.x == 0) {
@@ -313,6 +333,10 @@ void free_batched(hpccg_ctx *c)
     hipFree(B->colindex);
     hipFree(B->values);
     hipFree(B->row_off);
+    hipFree(B->slice_ptr);
+    hipFree(B->scol);
+    hipFree(B->sval);
+    hipFree(B->slice_off);
     hipFree(B->r);
     hipFree(B->p);
     hipFree(B->Ap);
@@ -377,17 +401,18 @@ static void enqueue_batch(hpccg_ctx *c, Batch *B, const BatchLayout &L, const do
                           double rel_tol2)
 {
     const dim3 grid(L.G), blk(ROWS);
-    hipLaunchKernelGGL((kb_true<ROWS, true>), grid, blk, 0, c->stream, B->rowindex, B->colindex, B->values, x, b, B->r,
-                       B->row_off, L.wg_team, L.team_first, L.team_size, B->st, B->partials, rel_tol2);
+    const BatchSell A = batch_sell(B);
+    hipLaunchKernelGGL((kb_true<ROWS, true>), grid, blk, 0, c->stream, A, x, b, B->r, B->row_off, L.wg_team,
+                       L.team_first, L.team_size, B->st, B->partials, rel_tol2);
     for (int it = 0; it < max_iter; ++it) {
         hipLaunchKernelGGL(kb_update_p<ROWS>, grid, blk, 0, c->stream, B->r, B->p, B->row_off, L.wg_team,
                            L.team_first, L.team_size, B->st);
-        hipLaunchKernelGGL(kb_spmv<ROWS>, grid, blk, 0, c->stream, B->rowindex, B->colindex, B->values, B->p, B->Ap,
-                           B->row_off, L.wg_team, L.team_first, L.team_size, B->st, B->partials);
+        hipLaunchKernelGGL(kb_spmv<ROWS>, grid, blk, 0, c->stream, A, B->p, B->Ap, B->row_off, L.wg_team,
+                           L.team_first, L.team_size, B->st, B->partials);
         hipLaunchKernelGGL(kb_update_xr<ROWS>, grid, blk, 0, c->stream, x, B->r, B->p, B->Ap, B->row_off, L.wg_team,
                            L.team_first, L.team_size, B->st, B->partials);
-        hipLaunchKernelGGL((kb_true<ROWS, false>), grid, blk, 0, c->stream, B->rowindex, B->colindex, B->values, x, b,
-                           B->r, B->row_off, L.wg_team, L.team_first, L.team_size, B->st, B->partials, rel_tol2);
+        hipLaunchKernelGGL((kb_true<ROWS, false>), grid, blk, 0, c->stream, A, x, b, B->r, B->row_off, L.wg_team,
+                           L.team_first, L.team_size, B->st, B->partials, rel_tol2);
     }
 }
 
@@ -499,6 +524,37 @@ int hpccg_batch_setup(hpccg_ctx *c, int32_t count, const int32_t *dims)
     hipLaunchKernelGGL(kb_set_last, dim3(1), dim3(1), 0, c->stream, B->rowindex, B->E, B->nnz);
     HPCCG_HIP_TRY(hipGetLastError());
     HPCCG_HIP_TRY(hipStreamSynchronize(c->stream));
+    // sliced copy, system by system (each system's rows start a new slice)
+    std::vector<int32_t> rowh(B->E + 1);
+    HPCCG_HIP_TRY(hipMemcpy(rowh.data(), B->rowindex, (B->E + 1) * sizeof(int32_t), hipMemcpyDeviceToHost));
+    std::vector<int> soff(count + 1, 0);
+    for (int s = 0; s < count; ++s)
+        soff[s + 1] = soff[s] + (int)((B->row_off_h[s + 1] - B->row_off_h[s] + kSlice - 1) / kSlice);
+    std::vector<int64_t> sp(soff[count] + 1, 0);
+    for (int s = 0; s < count; ++s)
+        sell_slice_ptr(rowh.data() + B->row_off_h[s], B->row_off_h[s + 1] - B->row_off_h[s], sp[soff[s]],
+                       sp.data() + soff[s]);
+    B->slots = sp[soff[count]];
+    if (hipMalloc(&B->slice_ptr, sp.size() * sizeof(int64_t)) != hipSuccess ||
+        hipMalloc(&B->scol, std::max<int64_t>(B->slots, 1) * sizeof(int32_t)) != hipSuccess ||
+        hipMalloc(&B->sval, std::max<int64_t>(B->slots, 1) * sizeof(__half)) != hipSuccess ||
+        hipMalloc(&B->slice_off, (count + 1) * sizeof(int)) != hipSuccess) {
+        free_batched(c);
+        set_error("batched: device allocation of the sliced copy failed");
+        return HPCCG_ERR_NOMEM;
+    }
+    HPCCG_HIP_TRY(hipMemcpy(B->slice_ptr, sp.data(), sp.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    HPCCG_HIP_TRY(hipMemcpy(B->slice_off, soff.data(), (count + 1) * sizeof(int), hipMemcpyHostToDevice));
+    for (int s = 0; s < count; ++s) {
+        const int ns = soff[s + 1] - soff[s];
+        const int64_t threads = (int64_t)ns * kSlice;
+        hipLaunchKernelGGL((k_csr_to_sell<kGroupF16, __half, __half>), dim3((unsigned)((threads + 255) / 256)),
+                           dim3(256), 0, c->stream, B->rowindex + B->row_off_h[s], B->colindex, B->values,
+                           B->slice_ptr + soff[s], B->scol, B->sval, B->row_off_h[s + 1] - B->row_off_h[s], ns,
+                           nullptr);
+    }
+    HPCCG_HIP_TRY(hipGetLastError());
+    HPCCG_HIP_TRY(hipStreamSynchronize(c->stream));
     return 0;
 }
 
@@ -535,8 +591,8 @@ int hpccg_batch_spmv_dev(hpccg_ctx *c, const float *x, float *y)
         s.rtrans = 1.0;
     }
     HPCCG_HIP_TRY(hipMemcpyAsync(B->st, init.data(), B->S * sizeof(BatchState), hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(kb_spmv<128>, dim3(L.G), dim3(128), 0, c->stream, B->rowindex, B->colindex, B->values, x, y,
-                       B->row_off, L.wg_team, L.team_first, L.team_size, B->st, B->partials);
+    hipLaunchKernelGGL(kb_spmv<128>, dim3(L.G), dim3(128), 0, c->stream, batch_sell(B), x, y, B->row_off, L.wg_team,
+                       L.team_first, L.team_size, B->st, B->partials);
     HPCCG_HIP_TRY(hipGetLastError());
     HPCCG_HIP_TRY(hipStreamSynchronize(c->stream));
     return 0;

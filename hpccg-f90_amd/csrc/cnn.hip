@@ -424,33 +424,42 @@ struct FcgState {
     unsigned counter[4];
 };
 
-// r = b - A x (FP32 store, FP64 arithmetic), r.r; PREP: also decides the first stop
-template <int ROWS, bool PREP>
-__global__ __launch_bounds__(ROWS) void kf_true(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
-                                                const __half *__restrict__ val, const double *__restrict__ x,
-                                                const double *__restrict__ b, float *__restrict__ r, int nrow,
-                                                int ntiles, FcgState *st, double *hist_true, double *partials)
+// r = b - A x (FP32 store, FP64 arithmetic) and the workgroup's share of r.r
+template <bool PREP>
+__device__ __forceinline__ bool kf_true_due(const FcgState *st)
 {
-    __shared__ double lds[ROWS * 27];
-    __shared__ double red[ROWS / kWave];
-    const int k = PREP ? 0 : st->k;
-    if (!PREP && (k >= st->stop_at || !st->need_true)) return;
-    int t0, t1, lb;
-    block_range(ntiles, t0, t1, lb);
+    return PREP || (st->k < st->stop_at && st->need_true);
+}
+
+template <bool PREP>
+__global__ __launch_bounds__(kSellThreads) void kf_true(const SellRef<__half> A, const double *__restrict__ x,
+                                                        const double *__restrict__ b, float *__restrict__ r,
+                                                        const FcgState *st, double *red_part)
+{
+    __shared__ double red[kSellThreads / kWave];
+    if (!kf_true_due<PREP>(st)) return;
+    int64_t row;
+    bool valid;
+    const double ax = sell_rowsum_of_thread<kGroupF16>(A, x, row, valid);
     double dot = 0.0;
-    for (int tile = t0; tile < t1; ++tile) {
-        bool valid;
-        int row;
-        const double ax = spmv_tile_rowsum<ROWS, __half, double>(rowptr, col, val, x, nrow, tile, lds, valid, row);
-        if (valid) {
-            const float ri = (float)(b[row] - ax);
-            r[row] = ri;
-            dot = fma((double)ri, (double)ri, dot);
-        }
+    if (valid) {
+        const float ri = (float)(b[row] - ax);
+        r[row] = ri;
+        dot = (double)ri * (double)ri;
     }
-    dot = block_sum<ROWS>(dot, red);
-    double rr;
-    if (grid_sum<ROWS>(dot, partials, &st->counter[0], gridDim.x, lb, red, &rr) && threadIdx.x == 0) {
+    store_partial(block_sum<kSellThreads>(dot, red), red_part);
+}
+
+// r.r of the true residual: network scales, stopping test; PREP also decides the first stop
+template <bool PREP>
+__global__ __launch_bounds__(kFinishThreads) void kf_true_finish(const double *__restrict__ red_part, int nparts,
+                                                                 int nrow, FcgState *st, double *hist_true)
+{
+    __shared__ double red[kFinishThreads / kWave];
+    if (!kf_true_due<PREP>(st)) return;
+    const int k = PREP ? 0 : st->k;
+    const double rr = sum_partials(red_part, nparts, red);
+    if (threadIdx.x == 0) {
         hist_true[k] = rr;
         st->rr = rr;
         st->rr_true = rr;
@@ -470,35 +479,35 @@ __global__ __launch_bounds__(ROWS) void kf_true(const int32_t *__restrict__ rowp
     }
 }
 
-// Ap = A p, p.Ap -> alpha = (r.z) / (p.Ap)
-template <int ROWS>
-__global__ __launch_bounds__(ROWS) void kf_spmv(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
-                                                const __half *__restrict__ val, const float *__restrict__ p,
-                                                float *__restrict__ Ap, int nrow, int ntiles, FcgState *st,
-                                                double *partials)
+// Ap = A p and the workgroup's share of p.Ap
+__global__ __launch_bounds__(kSellThreads) void kf_spmv(const SellRef<__half> A, const float *__restrict__ p,
+                                                        float *__restrict__ Ap, const FcgState *st, double *red_part)
 {
-    __shared__ double lds[ROWS * 27];
-    __shared__ double red[ROWS / kWave];
+    __shared__ double red[kSellThreads / kWave];
     if (st->k >= st->stop_at) return;
-    int t0, t1, lb;
-    block_range(ntiles, t0, t1, lb);
+    int64_t row;
+    bool valid;
+    const double s = sell_rowsum_of_thread<kGroupF16>(A, p, row, valid);
     double dot = 0.0;
-    for (int tile = t0; tile < t1; ++tile) {
-        bool valid;
-        int row;
-        const double s = spmv_tile_rowsum<ROWS, __half, float>(rowptr, col, val, p, nrow, tile, lds, valid, row);
-        if (valid) {
-            const float a = (float)s;
-            Ap[row] = a;
-            dot = fma((double)p[row], (double)a, dot);
-        }
+    if (valid) {
+        const float a = (float)s;
+        Ap[row] = a;
+        dot = (double)p[row] * (double)a;
     }
-    dot = block_sum<ROWS>(dot, red);
-    double pap;
-    if (grid_sum<ROWS>(dot, partials, &st->counter[1], gridDim.x, lb, red, &pap) && threadIdx.x == 0) {
+    store_partial(block_sum<kSellThreads>(dot, red), red_part);
+}
+
+// p.Ap -> alpha = (r.z) / (p.Ap); a breakdown stops before x is touched
+__global__ __launch_bounds__(kFinishThreads) void kf_spmv_finish(const double *__restrict__ red_part, int nparts,
+                                                                 FcgState *st)
+{
+    __shared__ double red[kFinishThreads / kWave];
+    if (st->k >= st->stop_at) return;
+    const double pap = sum_partials(red_part, nparts, red);
+    if (threadIdx.x == 0) {
         st->pap = pap;
         st->alpha = st->rz / pap;
-        if (!(pap > 0.0) || !isfinite(pap) || !isfinite(st->alpha)) {  // stop before x is touched
+        if (!(pap > 0.0) || !isfinite(pap) || !isfinite(st->alpha)) {
             st->breakdown = 1;
             st->stop_at = st->k;
             st->done = 1;
@@ -995,8 +1004,8 @@ int hpccg_fcg_learned_dev(hpccg_ctx *c, int32_t nx, int32_t ny, int32_t nz, cons
         set_error("no network: call hpccg_cnn_init / hpccg_cnn_set_weights / hpccg_cnn_train");
         return HPCCG_ERR_STATE;
     }
-    HPCCG_TRY(ensure_values16(c));
-    const int n = c->A.nrow, nt = (n + kTileRows - 1) / kTileRows;
+    HPCCG_TRY(ensure_sell16(c));
+    const int n = c->A.nrow;
     const Grid3 g = make_grid(nx, ny, nz);
     HPCCG_TRY(ensure_act(c, g.nvox(), 1));
     if (!fcg_of(c)) c->ext[2] = new FcgWs();
@@ -1044,17 +1053,21 @@ int hpccg_fcg_learned_dev(hpccg_ctx *c, int32_t nx, int32_t ny, int32_t nz, cons
     std::vector<double> neg(2 * (size_t)F->hist_cap, -1.0);
     HPCCG_HIP_TRY(hipMemcpyAsync(F->hist, neg.data(), neg.size() * 8, hipMemcpyHostToDevice, c->stream));
     double *hrec = F->hist, *htrue = F->hist + F->hist_cap;
-    const int Gs = HPCCG_SPMV_GRID(kf_spmv<kTileRows>, c->info.num_cu, nt), Gv = vec_grid_for(c->info.num_cu, n);
+    const Matrix &A = c->A;
+    const int Gs = sell_grid(A), Gv = vec_grid_for(c->info.num_cu, n);
     const int *skip = &F->st->done;
     const float *s_in = &F->st->s_in, *s_out = &F->st->s_out;
 
     auto iteration = [&]() {
-        hipLaunchKernelGGL(kf_spmv<kTileRows>, dim3(Gs), dim3(kTileRows), 0, c->stream, c->A.rowindex, c->A.colindex,
-                           c->A.values16, F->p, F->Ap, n, nt, F->st, c->partials);
+        hipLaunchKernelGGL(kf_spmv, dim3(Gs), dim3(kSellThreads), 0, c->stream, sell16(A), F->p, F->Ap, F->st,
+                           A.red_part);
+        hipLaunchKernelGGL(kf_spmv_finish, dim3(1), dim3(kFinishThreads), 0, c->stream, A.red_part, Gs, F->st);
         hipLaunchKernelGGL(kf_update_xr, dim3(Gv), dim3(kVecThreads), 0, c->stream, x, F->r, F->p, F->Ap, n, F->st,
                            hrec, c->partials);
-        hipLaunchKernelGGL((kf_true<kTileRows, false>), dim3(Gs), dim3(kTileRows), 0, c->stream, c->A.rowindex,
-                           c->A.colindex, c->A.values16, x, b, F->r, n, nt, F->st, htrue, c->partials);
+        hipLaunchKernelGGL(kf_true<false>, dim3(Gs), dim3(kSellThreads), 0, c->stream, sell16(A), x, b, F->r, F->st,
+                           A.red_part);
+        hipLaunchKernelGGL(kf_true_finish<false>, dim3(1), dim3(kFinishThreads), 0, c->stream, A.red_part, Gs, n,
+                           F->st, htrue);
         forward_fused(c, F->r, s_in, F->z, s_out, g, skip);
         hipLaunchKernelGGL(kf_dots<false>, dim3(Gv), dim3(kVecThreads), 0, c->stream, F->z, F->Ap, F->r, n, F->st,
                            c->partials);
@@ -1063,8 +1076,10 @@ int hpccg_fcg_learned_dev(hpccg_ctx *c, int32_t nx, int32_t ny, int32_t nz, cons
 
     HPCCG_HIP_TRY(hipEventRecord(c->ev0, c->stream));
     // Prepare: r = b - A x, z = M(r), rz = z.r, p = z
-    hipLaunchKernelGGL((kf_true<kTileRows, true>), dim3(Gs), dim3(kTileRows), 0, c->stream, c->A.rowindex,
-                       c->A.colindex, c->A.values16, x, b, F->r, n, nt, F->st, htrue, c->partials);
+    hipLaunchKernelGGL(kf_true<true>, dim3(Gs), dim3(kSellThreads), 0, c->stream, sell16(A), x, b, F->r, F->st,
+                       A.red_part);
+    hipLaunchKernelGGL(kf_true_finish<true>, dim3(1), dim3(kFinishThreads), 0, c->stream, A.red_part, Gs, n, F->st,
+                       htrue);
     forward_fused(c, F->r, s_in, F->z, s_out, g, skip);
     hipLaunchKernelGGL(kf_dots<true>, dim3(Gv), dim3(kVecThreads), 0, c->stream, F->z, F->Ap, F->r, n, F->st,
                        c->partials);

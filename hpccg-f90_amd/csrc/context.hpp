@@ -8,6 +8,7 @@
 #include <tuple>
 
 #include "common.hpp"
+#include "spmv.hpp"
 
 namespace hpccg {
 
@@ -29,13 +30,32 @@ struct Matrix {
     int32_t nrow = 0, ncol = 0;
     int64_t nnz = 0;
     int32_t max_row_nnz = 0;
-    int32_t max_tile_nnz = 0;  // max nnz over row tiles of kTileRows rows
-    int32_t *rowindex = nullptr;  // 1-based, nrow+1
+    int32_t *rowindex = nullptr;  // 1-based, nrow+1 (the reference's CSR, kept on the device)
     int32_t *colindex = nullptr;  // 1-based, nnz
     double *values = nullptr;
-    __half *values16 = nullptr;   // FP16 copy for the mixed-precision path (lazy)
     int32_t grid[3] = {0, 0, 0};  // nx, ny, nz when built by hpccg_generate_matrix
+
+    // sliced-ELL copies read by every SpMV kernel (layout in spmv.hpp)
+    int32_t nslices = 0;          // ceil(nrow / 64)
+    int64_t slots = 0;            // slots of one copy (>= nnz; padding of short rows)
+    int64_t *slice_ptr = nullptr; // nslices+1 slot offsets, shared by both copies
+    int32_t *scol64 = nullptr;    // FP64 copy: 0-based columns, 2 entries per lane group
+    double *sval64 = nullptr;
+    int32_t *scol16 = nullptr;    // FP16 copy (mixed / learned paths, built on first use)
+    __half *sval16 = nullptr;
+    double *red_part = nullptr;   // per-workgroup partials of the one-wave-per-slice kernels
 };
+
+inline SellRef<double> sell64(const Matrix &A)
+{
+    return SellRef<double>{A.slice_ptr, A.scol64, A.sval64, A.rowindex, A.nrow, A.nslices};
+}
+inline SellRef<__half> sell16(const Matrix &A)
+{
+    return SellRef<__half>{A.slice_ptr, A.scol16, A.sval16, A.rowindex, A.nrow, A.nslices};
+}
+// workgroups of a one-wave-per-slice launch (at least one, so reductions always run)
+inline int sell_grid(const Matrix &A) { return A.nslices > 0 ? sell_blocks(A.nslices) : 1; }
 
 struct GraphKey {
     const void *b, *x;
@@ -94,7 +114,7 @@ void free_batched(hpccg_ctx *c);
 void free_learned(hpccg_ctx *c);
 // drop the learned path's captured graphs (they hold matrix and activation addresses)
 void invalidate_learned_graphs(hpccg_ctx *c);
-int ensure_values16(hpccg_ctx *c);
+int ensure_sell16(hpccg_ctx *c);
 int cg_device_fp64(hpccg_ctx *c, const double *b, double *x, int max_iter, double tol, int *niters,
                    double *normr, double *hist, double *elapsed_ms);
 // FP64 cg (tree dots) on device arrays without graphs, calling hook(k) after the residual

@@ -41,51 +41,45 @@ struct MixState {
     unsigned counter[4];
 };
 
-// ---- FP16 values -------------------------------------------------------------------------
-__global__ void k_to_half(const double *__restrict__ v, __half *__restrict__ h, int64_t nnz, int *inexact)
+// ---- kernels --------------------------------------------------------------------------------
+// True residual r = b - A x (FP16 values, FP64 x, FP64 accumulation, FP32 store) and the
+// workgroup's share of r.r.  PREP: Prepare (iteration 0).  Otherwise runs only when the
+// previous kernel requested it.
+template <bool PREP>
+__device__ __forceinline__ bool km_true_due(const MixState *st)
 {
-    int bad = 0;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nnz; i += (int64_t)gridDim.x * blockDim.x) {
-        const __half q = __float2half((float)v[i]);  // exact whenever v is an FP16 value
-        h[i] = q;
-        if ((double)__half2float(q) != v[i]) bad = 1;
-    }
-    if (bad) atomicOr(inexact, 1);
+    return PREP || (st->k < st->stop_at && st->need_true);
 }
 
-// ---- kernels --------------------------------------------------------------------------------
-// True residual r = b - A x (FP16 values, FP64 x, FP64 accumulation, FP32 store) and r.r.
-// PREP: Prepare (iteration 0).  Otherwise runs only when the previous kernel requested it.
-template <int ROWS, bool PREP>
-__global__ __launch_bounds__(ROWS) void km_true_residual(const int32_t *__restrict__ rowptr,
-                                                         const int32_t *__restrict__ col,
-                                                         const __half *__restrict__ val,
-                                                         const double *__restrict__ x,
-                                                         const double *__restrict__ b,
-                                                         float *__restrict__ r, int nrow, int ntiles,
-                                                         MixState *st, double *hist_true,
-                                                         double *partials)
+template <bool PREP>
+__global__ __launch_bounds__(kSellThreads) void km_true_residual(const SellRef<__half> A, const double *__restrict__ x,
+                                                                 const double *__restrict__ b, float *__restrict__ r,
+                                                                 const MixState *st, double *red_part)
 {
-    __shared__ double lds[ROWS * 27];
-    __shared__ double red[ROWS / kWave];
-    const int k = PREP ? 0 : st->k;
-    if (!PREP && (k >= st->stop_at || !st->need_true)) return;
-    int t0, t1, lb;
-    block_range(ntiles, t0, t1, lb);
+    __shared__ double red[kSellThreads / kWave];
+    if (!km_true_due<PREP>(st)) return;
+    int64_t row;
+    bool valid;
+    const double ax = sell_rowsum_of_thread<kGroupF16>(A, x, row, valid);
     double dot = 0.0;
-    for (int tile = t0; tile < t1; ++tile) {
-        bool valid;
-        int row;
-        const double ax = spmv_tile_rowsum<ROWS, __half, double>(rowptr, col, val, x, nrow, tile, lds, valid, row);
-        if (valid) {
-            const float ri = (float)(b[row] - ax);
-            r[row] = ri;
-            dot = fma((double)ri, (double)ri, dot);
-        }
+    if (valid) {
+        const float ri = (float)(b[row] - ax);
+        r[row] = ri;
+        dot = (double)ri * (double)ri;
     }
-    dot = block_sum<ROWS>(dot, red);
-    double rr;
-    if (grid_sum<ROWS>(dot, partials, &st->counter[0], gridDim.x, lb, red, &rr) && threadIdx.x == 0) {
+    store_partial(block_sum<kSellThreads>(dot, red), red_part);
+}
+
+// r.r of the true residual: replacement, convergence test on the FP64 true residual, beta.
+template <bool PREP>
+__global__ __launch_bounds__(kFinishThreads) void km_true_finish(const double *__restrict__ red_part, int nparts,
+                                                                 MixState *st, double *hist_true)
+{
+    __shared__ double red[kFinishThreads / kWave];
+    if (!km_true_due<PREP>(st)) return;
+    const int k = PREP ? 0 : st->k;
+    const double rr = sum_partials(red_part, nparts, red);
+    if (threadIdx.x == 0) {
         hist_true[k] = rr;
         const bool conv = sqrt(rr) < st->tol;
         if (PREP) {
@@ -125,33 +119,32 @@ __global__ __launch_bounds__(kVecThreads) void km_update_p(const float *__restri
     for (int64_t i = i0 + threadIdx.x; i < i1; i += kVecThreads) p[i] = (float)fma(beta, (double)p[i], (double)r[i]);
 }
 
-// Ap = A p (FP16 values, FP32 p, FP64 accumulation, FP32 store) and p.Ap -> alpha.
-template <int ROWS>
-__global__ __launch_bounds__(ROWS) void km_spmv(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
-                                                const __half *__restrict__ val, const float *__restrict__ p,
-                                                float *__restrict__ Ap, int nrow, int ntiles, MixState *st,
-                                                double *partials)
+// Ap = A p (FP16 values, FP32 p, FP64 accumulation, FP32 store) and the workgroup's p.Ap.
+__global__ __launch_bounds__(kSellThreads) void km_spmv(const SellRef<__half> A, const float *__restrict__ p,
+                                                        float *__restrict__ Ap, const MixState *st, double *red_part)
 {
-    __shared__ double lds[ROWS * 27];
-    __shared__ double red[ROWS / kWave];
-    const int k = st->k;
-    if (k >= st->stop_at) return;
-    int t0, t1, lb;
-    block_range(ntiles, t0, t1, lb);
+    __shared__ double red[kSellThreads / kWave];
+    if (st->k >= st->stop_at) return;
+    int64_t row;
+    bool valid;
+    const double s = sell_rowsum_of_thread<kGroupF16>(A, p, row, valid);
     double dot = 0.0;
-    for (int tile = t0; tile < t1; ++tile) {
-        bool valid;
-        int row;
-        const double s = spmv_tile_rowsum<ROWS, __half, float>(rowptr, col, val, p, nrow, tile, lds, valid, row);
-        if (valid) {
-            const float a = (float)s;
-            Ap[row] = a;
-            dot = fma((double)p[row], (double)a, dot);
-        }
+    if (valid) {
+        const float a = (float)s;
+        Ap[row] = a;
+        dot = (double)p[row] * (double)a;
     }
-    dot = block_sum<ROWS>(dot, red);
-    double pap;
-    if (grid_sum<ROWS>(dot, partials, &st->counter[1], gridDim.x, lb, red, &pap) && threadIdx.x == 0) {
+    store_partial(block_sum<kSellThreads>(dot, red), red_part);
+}
+
+// p.Ap -> alpha
+__global__ __launch_bounds__(kFinishThreads) void km_spmv_finish(const double *__restrict__ red_part, int nparts,
+                                                                 MixState *st)
+{
+    __shared__ double red[kFinishThreads / kWave];
+    if (st->k >= st->stop_at) return;
+    const double pap = sum_partials(red_part, nparts, red);
+    if (threadIdx.x == 0) {
         if (!(pap != 0.0) || !isfinite(pap)) st->breakdown = 1;
         st->alpha = st->rtrans / pap;
     }
@@ -196,24 +189,43 @@ __global__ __launch_bounds__(kVecThreads) void km_update_xr(double *__restrict__
 }
 
 // ---- host ------------------------------------------------------------------------------------
-int ensure_values16(hpccg_ctx *c)
+// FP16 sliced copy of the matrix (built on first use): every value must be an FP16 number.
+int ensure_sell16(hpccg_ctx *c)
 {
-    if (c->A.values16) return 0;
-    const int64_t nnz = c->A.nnz;
-    HPCCG_HIP_TRY(hipMalloc(&c->A.values16, std::max<int64_t>(nnz, 1) * sizeof(__half)));
+    Matrix &A = c->A;
+    if (A.sval16) return 0;
+    if (hipMalloc(&A.scol16, std::max<int64_t>(A.slots, 1) * sizeof(int32_t)) != hipSuccess ||
+        hipMalloc(&A.sval16, std::max<int64_t>(A.slots, 1) * sizeof(__half)) != hipSuccess) {
+        hipFree(A.scol16);
+        hipFree(A.sval16);
+        A.scol16 = nullptr;
+        A.sval16 = nullptr;
+        set_error("mixed precision: allocation of the FP16 sliced copy failed");
+        return HPCCG_ERR_NOMEM;
+    }
     int *flag = nullptr;
     HPCCG_HIP_TRY(hipMalloc(&flag, sizeof(int)));
-    int zero = 0, bad = 0;
-    HPCCG_HIP_TRY(hipMemcpyAsync(flag, &zero, sizeof(int), hipMemcpyHostToDevice, c->stream));
-    if (nnz > 0)
-        hipLaunchKernelGGL(k_to_half, dim3(c->info.num_cu * 8), dim3(256), 0, c->stream, c->A.values, c->A.values16, nnz,
-                           flag);
-    HPCCG_HIP_TRY(hipMemcpyAsync(&bad, flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    HPCCG_HIP_TRY(hipStreamSynchronize(c->stream));
+    int bad = 0;
+    hipError_t e = hipMemsetAsync(flag, 0, sizeof(int), c->stream);
+    if (e == hipSuccess && A.nslices > 0) {
+        const int64_t threads = (int64_t)A.nslices * kSlice;
+        hipLaunchKernelGGL((k_csr_to_sell<kGroupF16, __half>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0,
+                           c->stream, A.rowindex, A.colindex, A.values, A.slice_ptr, A.scol16, A.sval16,
+                           (int64_t)A.nrow, A.nslices, flag);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&bad, flag, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     hipFree(flag);
-    if (bad) {
-        hipFree(c->A.values16);
-        c->A.values16 = nullptr;
+    if (e != hipSuccess || bad) {
+        hipFree(A.scol16);
+        hipFree(A.sval16);
+        A.scol16 = nullptr;
+        A.sval16 = nullptr;
+        if (e != hipSuccess) {
+            set_error("mixed precision: FP16 copy: %s", hipGetErrorString(e));
+            return HPCCG_ERR_HIP;
+        }
         set_error("mixed precision: some matrix values are not exactly representable in FP16");
         return HPCCG_ERR_ARG;
     }
@@ -270,21 +282,41 @@ static int ensure_mixed_ws(hpccg_ctx *c, int64_t n, int max_iter)
     return 0;
 }
 
+// Ap = A p with p.Ap -> alpha; true residual r = b - A x with its epilogue
+static void launch_km_spmv(hpccg_ctx *c, const float *p, float *Ap)
+{
+    const Matrix &A = c->A;
+    const int G = sell_grid(A);
+    MixState *st = (MixState *)c->mix.state;
+    hipLaunchKernelGGL(km_spmv, dim3(G), dim3(kSellThreads), 0, c->stream, sell16(A), p, Ap, st, A.red_part);
+    hipLaunchKernelGGL(km_spmv_finish, dim3(1), dim3(kFinishThreads), 0, c->stream, A.red_part, G, st);
+}
+template <bool PREP>
+static void launch_km_true(hpccg_ctx *c, const double *b, const double *x)
+{
+    const Matrix &A = c->A;
+    const int G = sell_grid(A);
+    MixedWs &m = c->mix;
+    MixState *st = (MixState *)m.state;
+    hipLaunchKernelGGL(km_true_residual<PREP>, dim3(G), dim3(kSellThreads), 0, c->stream, sell16(A), x, b, m.r, st,
+                       A.red_part);
+    hipLaunchKernelGGL(km_true_finish<PREP>, dim3(1), dim3(kFinishThreads), 0, c->stream, A.red_part, G, st,
+                       m.hist + m.hist_cap);
+}
+
 static int mixed_enqueue(hpccg_ctx *c, const double *b, double *x, int count)
 {
     MixedWs &m = c->mix;
-    const int n = c->A.nrow, nt = (n + kTileRows - 1) / kTileRows;
-    const int Gs = HPCCG_SPMV_GRID(km_spmv<kTileRows>, c->info.num_cu, nt), Gv = vec_grid_for(c->info.num_cu, n);
+    const int n = c->A.nrow;
+    const int Gv = vec_grid_for(c->info.num_cu, n);
     MixState *st = (MixState *)m.state;
-    double *hrec = m.hist, *htrue = m.hist + m.hist_cap;
+    double *hrec = m.hist;
     for (int it = 0; it < count; ++it) {
         hipLaunchKernelGGL(km_update_p, dim3(Gv), dim3(kVecThreads), 0, c->stream, m.r, m.p, n, st);
-        hipLaunchKernelGGL(km_spmv<kTileRows>, dim3(Gs), dim3(kTileRows), 0, c->stream, c->A.rowindex, c->A.colindex,
-                           c->A.values16, m.p, m.Ap, n, nt, st, c->partials);
+        launch_km_spmv(c, m.p, m.Ap);
         hipLaunchKernelGGL(km_update_xr, dim3(Gv), dim3(kVecThreads), 0, c->stream, x, m.r, m.p, m.Ap, n, st, hrec,
                            c->partials);
-        hipLaunchKernelGGL((km_true_residual<kTileRows, false>), dim3(Gs), dim3(kTileRows), 0, c->stream,
-                           c->A.rowindex, c->A.colindex, c->A.values16, x, b, m.r, n, nt, st, htrue, c->partials);
+        launch_km_true<false>(c, b, x);
     }
     HPCCG_HIP_TRY(hipGetLastError());
     return 0;
@@ -321,8 +353,8 @@ static int cg_mixed_device(hpccg_ctx *c, const double *b, double *x, int max_ite
         return HPCCG_ERR_STATE;
     }
     HPCCG_CHECK_ARG(max_iter >= 0 && replace_every >= 0, "max_iter and replace_every must be >= 0");
-    const int n = c->A.nrow, nt = (n + kTileRows - 1) / kTileRows;
-    HPCCG_TRY(ensure_values16(c));
+    const int n = c->A.nrow;
+    HPCCG_TRY(ensure_sell16(c));
     HPCCG_TRY(ensure_mixed_ws(c, n, max_iter));
     MixedWs &m = c->mix;
     MixState init{};
@@ -341,9 +373,7 @@ static int cg_mixed_device(hpccg_ctx *c, const double *b, double *x, int max_ite
     if (max_iter > 0) HPCCG_TRY(mixed_graph(c, b, x, chunk, &ge));
 
     HPCCG_HIP_TRY(hipEventRecord(c->ev0, c->stream));
-    hipLaunchKernelGGL((km_true_residual<kTileRows, true>), dim3(HPCCG_SPMV_GRID((km_true_residual<kTileRows, true>), c->info.num_cu, nt)),
-                       dim3(kTileRows), 0, c->stream, c->A.rowindex, c->A.colindex, c->A.values16, x, b, m.r, n, nt, (MixState *)m.state,
-                       m.hist + m.hist_cap, c->partials);
+    launch_km_true<true>(c, b, x);
     HPCCG_HIP_TRY(hipGetLastError());
     int launched = 0;
     MixState *sh = (MixState *)m.state_h;
@@ -446,7 +476,7 @@ int hpccg_spmv_mixed_dev(hpccg_ctx *c, const float *x, float *y)
         set_error("no matrix");
         return HPCCG_ERR_STATE;
     }
-    HPCCG_TRY(ensure_values16(c));
+    HPCCG_TRY(ensure_sell16(c));
     HPCCG_TRY(ensure_mixed_ws(c, c->A.nrow, 1));
     MixState init{};
     init.k = 1;
@@ -454,10 +484,7 @@ int hpccg_spmv_mixed_dev(hpccg_ctx *c, const float *x, float *y)
     init.rtrans = 1.0;
     std::memcpy(c->mix.state_h, &init, sizeof(init));
     HPCCG_HIP_TRY(hipMemcpyAsync(c->mix.state, c->mix.state_h, sizeof(MixState), hipMemcpyHostToDevice, c->stream));
-    const int n = c->A.nrow, nt = (n + kTileRows - 1) / kTileRows;
-    hipLaunchKernelGGL(km_spmv<kTileRows>, dim3(HPCCG_SPMV_GRID(km_spmv<kTileRows>, c->info.num_cu, nt)), dim3(kTileRows), 0, c->stream,
-                       c->A.rowindex, c->A.colindex, c->A.values16, x, y, n, nt, (MixState *)c->mix.state,
-                       c->partials);
+    launch_km_spmv(c, x, y);
     HPCCG_HIP_TRY(hipGetLastError());
     HPCCG_HIP_TRY(hipStreamSynchronize(c->stream));
     return 0;
@@ -467,12 +494,10 @@ int hpccg_time_spmv_mixed(hpccg_ctx *c, const float *x, float *y, int32_t reps, 
 {
     HPCCG_CHECK_ARG(c && x && y && reps > 0 && avg_ms, "bad argument");
     HPCCG_TRY(hpccg_spmv_mixed_dev(c, x, y));  // sets up state and warms
-    const int n = c->A.nrow, nt = (n + kTileRows - 1) / kTileRows;
-    const int G = HPCCG_SPMV_GRID(km_spmv<kTileRows>, c->info.num_cu, nt);
     HPCCG_HIP_TRY(hipEventRecord(c->ev0, c->stream));
     for (int i = 0; i < reps; ++i)
-        hipLaunchKernelGGL(km_spmv<kTileRows>, dim3(G), dim3(kTileRows), 0, c->stream, c->A.rowindex, c->A.colindex,
-                           c->A.values16, x, y, n, nt, (MixState *)c->mix.state, c->partials);
+        hipLaunchKernelGGL(km_spmv, dim3(sell_grid(c->A)), dim3(kSellThreads), 0, c->stream, sell16(c->A), x, y,
+                           (const MixState *)c->mix.state, c->A.red_part);
     HPCCG_HIP_TRY(hipGetLastError());
     HPCCG_HIP_TRY(hipEventRecord(c->ev1, c->stream));
     HPCCG_HIP_TRY(hipEventSynchronize(c->ev1));

@@ -1,4 +1,24 @@
-// CSR SpMV building blocks shared by the FP64, mixed-precision and batched paths.
+// CSR SpMV building blocks shared by the FP64, mixed-precision, batched and learned paths.
+//
+// The reference's CSR (1-based rowindex / colindex / values, SparseMatrix.F90:44-64) is the
+// interface; on the device every SpMV reads a sliced-ELL copy of it built once per upload:
+//
+//   * rows are cut into slices of 64 consecutive rows (one wavefront, one lane per row);
+//   * slice s is K_s entries wide (K_s = its longest row) and occupies the slot range
+//     [slice_ptr[s], slice_ptr[s] + 64 K_s);
+//   * inside a slice the entries are stored "column-major" in groups of G per lane: entry j of
+//     the slice's lane l sits at slot  base + ((j / G) * 64 + l) * G + j % G  for the first
+//     (K_s / G) * G entries and at  base + KG * 64 + (j - KG) * 64 + l  after them (KG =
+//     (K_s / G) * G), so that one load instruction of the wave reads 64 * G consecutive
+//     entries (fully coalesced) and each lane gets G consecutive entries of its own row;
+//   * columns are stored 0-based; slots past a row's end hold value 0 and a valid column.
+//
+// Each lane sums its own row in ascending entry order from 0.0 and skips the padding slots
+// (the row length comes from rowindex), so y = A x is bitwise the reference's row loop
+// (SparseMatrix.F90:225-229) for FP64 values -- and for FP16 values it is the FP64 row sum
+// of the exactly widened products.  Compared with a CSR kernel this layout needs no LDS
+// staging and no per-row scatter: the value and column streams are read with unit stride
+// and the x gathers of neighbouring lanes touch neighbouring rows' columns.
 #pragma once
 
 #include <hip/hip_fp16.h>
@@ -7,13 +27,14 @@
 
 namespace hpccg {
 
-constexpr int kTileRows = 64;  // rows per SpMV tile == threads per SpMV workgroup (one wave)
+constexpr int kSlice = 64;            // rows per slice == lanes per wave
+constexpr int kSellWaves = 4;         // slices per SpMV workgroup (one per wave)
+constexpr int kSellThreads = kSlice * kSellWaves;
+constexpr int kStencilK = 27;         // slice width of the 27-point stencil: fully unrolled path
+constexpr int kGroupF64 = 2;          // FP64 copy: 2 entries per lane per load (16-byte value loads)
+constexpr int kGroupF16 = 1;          // FP16 copy: 1 entry per lane per load (measured best)
 constexpr int kVecThreads = 256;
-constexpr int kMaxGrid = 16384;
-
-// FP64 SpMV kernels: LDS (27 FP64 products per row, 13.8 KB per 64-row tile) admits 11
-// workgroups per CU, i.e. 3 waves per SIMD; cap the registers so they do not cut that to 2.
-#define HPCCG_SPMV_OCC __attribute__((amdgpu_waves_per_eu(3)))
+constexpr int kMaxGrid = 16384;       // partial slots of the vector kernels' grid reductions
 
 __device__ __forceinline__ double to_f64(double v) { return v; }
 __device__ __forceinline__ double to_f64(float v) { return (double)v; }
@@ -27,92 +48,187 @@ __device__ __forceinline__ __half ld_stream(const __half *p)
     return __ushort_as_half(__builtin_nontemporal_load(reinterpret_cast<const unsigned short *>(p)));
 }
 
-// One tile of a CSR SpMV: rows [tile*ROWS, tile*ROWS+ROWS) of a 1-based CSR.
-//
-// Phase 1 streams the tile's contiguous (value, column) range with unit-stride (coalesced)
-// loads, gathers x, and stages the products, formed and rounded in FP64, in LDS.  Phase 2
-// sums each row's products in ascending k from 0.0 (the reference's order,
-// SparseMatrix.F90:225-229).  Tiles whose nnz exceed the LDS capacity (rows longer than 27 on
-// average; never the case for generate_matrix's CSR) take a direct per-row loop with the same
-// order.  VT = stored value type (double, or __half holding {27,-1} exactly), XT = vector
-// type.  With VT = double and XT = double the result is bitwise the reference's spmv.
-// Returns the row sum (FP64) of this thread's row, or 0 past the last row.
-//
-// The phase-1 loads are branch-free: a slot past the tile's last element re-reads that last
-// element (a valid address) and its product is dropped, so all UNR value and column loads of
-// a thread issue back to back.  UNR = 27 covers a full tile of the 27-point stencil in one pass.
-template <int ROWS, typename VT, typename XT, int UNR = 27>
-__device__ __forceinline__ double spmv_tile_rowsum(const int32_t *__restrict__ rowptr,
-                                                   const int32_t *__restrict__ col,
-                                                   const VT *__restrict__ val,
-                                                   const XT *__restrict__ x, int nrow, int tile,
-                                                   double *lds, bool &valid, int &row)
+typedef int i32x2_t __attribute__((ext_vector_type(2)));
+typedef double f64x2_t __attribute__((ext_vector_type(2)));
+
+// G consecutive entries of one lane (G = 1 or 2; the slot is G-aligned)
+template <int G>
+__device__ __forceinline__ void ld_group(const int32_t *p, int32_t *o)
 {
-    const int t = threadIdx.x;
-    const int r0 = tile * ROWS;
-    const int r = r0 + t;
-    const int rlast = min(r0 + ROWS, nrow);
-    const int base = rowptr[r0] - 1;
-    const int end = rowptr[rlast] - 1;
-    int ms = 0, me = 0;
-    valid = r < nrow;
-    row = r;
-    if (valid) {
-        ms = rowptr[r] - 1;
-        me = rowptr[r + 1] - 1;
-    }
-    double sum = 0.0;
-    if (end - base <= ROWS * 27) {
-        const int last = end - 1;  // only read when end > base
-        for (int k0 = base; k0 < end; k0 += UNR * ROWS) {
-            VT v[UNR];
-            int c[UNR];
-            if (sizeof(VT) == 8) {
-                // FP64 values: all column loads first (their gathers can start while the values
-                // are in flight; this order also needs fewer registers)
-#pragma unroll
-                for (int u = 0; u < UNR; ++u) c[u] = ld_stream(col + min(k0 + t + u * ROWS, last));
-#pragma unroll
-                for (int u = 0; u < UNR; ++u) v[u] = ld_stream(val + min(k0 + t + u * ROWS, last));
-            } else {
-#pragma unroll
-                for (int u = 0; u < UNR; ++u) {
-                    const int kk = min(k0 + t + u * ROWS, last);
-                    v[u] = ld_stream(val + kk);
-                    c[u] = ld_stream(col + kk);
-                }
-            }
-            // all gathers before the first LDS store: x is a generic pointer the compiler must
-            // assume may alias LDS, so an interleaved store would serialise the gathers
-            XT xg[UNR];
-#pragma unroll
-            for (int u = 0; u < UNR; ++u) xg[u] = x[c[u] - 1];
-#pragma unroll
-            for (int u = 0; u < UNR; ++u) {
-                const int kk = k0 + t + u * ROWS;
-                const double prod = to_f64(v[u]) * to_f64(xg[u]);
-                if (kk < end) lds[kk - base] = prod;
-            }
-        }
-        __syncthreads();
-        for (int j = ms; j < me; ++j) sum = sum + lds[j - base];
-        __syncthreads();
+    if constexpr (G == 1) {
+        o[0] = ld_stream(p);
     } else {
-        for (int j = ms; j < me; ++j) {
-            const double prod = to_f64(val[j]) * to_f64(x[col[j] - 1]);
-            sum = sum + prod;
+        static_assert(G == 2, "group of 1 or 2");
+        const i32x2_t v = __builtin_nontemporal_load(reinterpret_cast<const i32x2_t *>(p));
+        o[0] = v.x;
+        o[1] = v.y;
+    }
+}
+template <int G>
+__device__ __forceinline__ void ld_group(const double *p, double *o)
+{
+    if constexpr (G == 1) {
+        o[0] = ld_stream(p);
+    } else {
+        static_assert(G == 2, "group of 1 or 2");
+        const f64x2_t v = __builtin_nontemporal_load(reinterpret_cast<const f64x2_t *>(p));
+        o[0] = v.x;
+        o[1] = v.y;
+    }
+}
+template <int G>
+__device__ __forceinline__ void ld_group(const __half *p, __half *o)
+{
+    static_assert(G == 1, "FP16 copy uses single-entry groups");
+    o[0] = ld_stream(p);
+}
+
+// slot of entry j of lane `lane` in a slice of width K starting at slot `base`
+template <int G>
+__host__ __device__ __forceinline__ int64_t sell_slot(int64_t base, int K, int j, int lane)
+{
+    const int KG = (K / G) * G;
+    if (j < KG) return base + ((int64_t)(j / G) * kSlice + lane) * G + (j % G);
+    return base + (int64_t)KG * kSlice + (int64_t)(j - KG) * kSlice + lane;
+}
+
+// Row sum (FP64, ascending entry order from 0.0) of global row `row` held by lane `lane` of
+// slice `s`; 0 when !valid.  rowptr is the 1-based CSR row pointer (row lengths).
+template <int G, typename VT, typename XT>
+__device__ __forceinline__ double sell_rowsum(const int64_t *__restrict__ slice_ptr,
+                                              const int32_t *__restrict__ scol,
+                                              const VT *__restrict__ sval,
+                                              const int32_t *__restrict__ rowptr,
+                                              const XT *__restrict__ x, int s, int lane, int64_t row,
+                                              bool valid)
+{
+    const int len = valid ? rowptr[row + 1] - rowptr[row] : 0;
+    const int64_t base = slice_ptr[s];
+    const int K = (int)((slice_ptr[s + 1] - base) / kSlice);
+    double sum = 0.0;
+    if (K == kStencilK) {
+        // branch-free: all column and value loads issue back to back, then the gathers
+        constexpr int NG = kStencilK / G, KG = NG * G;
+        int32_t c[kStencilK];
+        VT v[kStencilK];
+#pragma unroll
+        for (int g = 0; g < NG; ++g) ld_group<G>(scol + base + ((int64_t)g * kSlice + lane) * G, c + g * G);
+#pragma unroll
+        for (int j = KG; j < kStencilK; ++j) c[j] = ld_stream(scol + base + (int64_t)j * kSlice + lane);
+#pragma unroll
+        for (int g = 0; g < NG; ++g) ld_group<G>(sval + base + ((int64_t)g * kSlice + lane) * G, v + g * G);
+#pragma unroll
+        for (int j = KG; j < kStencilK; ++j) v[j] = ld_stream(sval + base + (int64_t)j * kSlice + lane);
+#pragma unroll
+        for (int j = 0; j < kStencilK; ++j) {
+            const double prod = to_f64(v[j]) * to_f64(x[c[j]]);
+            sum = (j < len) ? sum + prod : sum;
+        }
+    } else {
+        for (int j = 0; j < K; ++j) {
+            const int64_t q = sell_slot<G>(base, K, j, lane);
+            const double prod = to_f64(sval[q]) * to_f64(x[scol[q]]);
+            sum = (j < len) ? sum + prod : sum;
         }
     }
     return sum;
 }
 
-// Contiguous tile range of one (XCD-renumbered) workgroup.
-__device__ __forceinline__ void block_range(int ntiles, int &t0, int &t1, int &lb)
+// A sliced-ELL copy as kernels take it (by value).
+template <typename VT>
+struct SellRef {
+    const int64_t *slice_ptr;  // nslices+1 slot offsets
+    const int32_t *col;        // 0-based columns
+    const VT *val;
+    const int32_t *rowptr;     // the 1-based CSR row pointer (row lengths)
+    int nrow, nslices;
+};
+
+// One wave per slice, kSellWaves slices per workgroup: the FP64 row sum of this thread's row
+// (0 past the last row or slice), with the row and whether it exists.
+template <int G, typename VT, typename XT>
+__device__ __forceinline__ double sell_rowsum_of_thread(const SellRef<VT> &A, const XT *__restrict__ x, int64_t &row,
+                                                        bool &valid)
 {
-    const int G = gridDim.x;
-    lb = xcd_logical_block(blockIdx.x, G);
-    t0 = (int)(((int64_t)lb * ntiles) / G);
-    t1 = (int)(((int64_t)(lb + 1) * ntiles) / G);
+    const int s = blockIdx.x * kSellWaves + __builtin_amdgcn_readfirstlane(threadIdx.x / kSlice);
+    const int lane = threadIdx.x % kSlice;
+    row = (int64_t)s * kSlice + lane;
+    valid = s < A.nslices && row < A.nrow;
+    if (s >= A.nslices) return 0.0;
+    return sell_rowsum<G, VT, XT>(A.slice_ptr, A.col, A.val, A.rowptr, x, s, lane, row, valid);
+}
+
+// CSR -> sliced ELL for the `nslices` slices of rows [0, nrow) of a 1-based CSR (rowptr,
+// col, val); the sliced copy stores 0-based columns.  One thread per slice row.  A VT
+// narrower than double raises *inexact for every value that does not convert exactly.
+template <int G, typename VT, typename VS = double>
+__global__ void k_csr_to_sell(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
+                              const VS *__restrict__ val, const int64_t *__restrict__ slice_ptr,
+                              int32_t *__restrict__ scol, VT *__restrict__ sval, int64_t nrow, int nslices,
+                              int *inexact)
+{
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nslices * kSlice) return;
+    const int s = (int)(t / kSlice), lane = (int)(t % kSlice);
+    const int64_t base = slice_ptr[s];
+    const int K = (int)((slice_ptr[s + 1] - base) / kSlice);
+    int len = 0, padc = 0, bad = 0;
+    if (t < nrow) {
+        const int64_t k0 = rowptr[t] - 1;
+        len = rowptr[t + 1] - rowptr[t];
+        for (int j = 0; j < len; ++j) {
+            const int64_t q = sell_slot<G>(base, K, j, lane);
+            const double v = to_f64(val[k0 + j]);
+            scol[q] = col[k0 + j] - 1;
+            const VT w = (VT)v;
+            sval[q] = w;
+            if (to_f64(w) != v) bad = 1;
+        }
+        padc = len > 0 ? col[k0 + len - 1] - 1 : 0;
+    }
+    for (int j = len; j < K; ++j) {
+        const int64_t q = sell_slot<G>(base, K, j, lane);
+        scol[q] = padc;
+        sval[q] = (VT)0.0;
+    }
+    if (bad && inexact) atomicOr(inexact, 1);
+}
+
+// Host: slot offsets of the slices of rows [0, nrow) of a 1-based CSR row pointer; returns
+// the total slot count.  sp must hold ceil(nrow/64)+1 entries; sp[0] = base.
+inline int64_t sell_slice_ptr(const int32_t *rowptr, int64_t nrow, int64_t base, int64_t *sp)
+{
+    const int64_t ns = (nrow + kSlice - 1) / kSlice;
+    sp[0] = base;
+    for (int64_t s = 0; s < ns; ++s) {
+        int K = 0;
+        const int64_t r1 = nrow < (s + 1) * kSlice ? nrow : (s + 1) * kSlice;
+        for (int64_t r = s * kSlice; r < r1; ++r) K = K > rowptr[r + 1] - rowptr[r] ? K : rowptr[r + 1] - rowptr[r];
+        sp[s + 1] = sp[s] + (int64_t)K * kSlice;
+    }
+    return sp[ns] - base;
+}
+
+// Grid reductions of the one-wave-per-slice kernels.  Those launch one workgroup per
+// kSellWaves slices (65,536 workgroups at 256^3), too many for a last-workgroup ticket: its
+// device-scope release fence writes back the L2 of the workgroup's XCD, and tens of thousands
+// of them cost more than the SpMV.  Instead each workgroup stores its total in part[blockIdx.x]
+// (plain store; the kernel boundary makes it visible) and a one-workgroup finish kernel sums
+// the partials in a fixed order -- deterministic, fence-free.
+constexpr int kFinishThreads = 1024;
+
+__device__ __forceinline__ void store_partial(double block_total, double *part)
+{
+    if (threadIdx.x == 0) part[blockIdx.x] = block_total;
+}
+
+// Sum of part[0..n) by the single workgroup of a finish kernel (kFinishThreads threads); every
+// thread gets the total.  `red` needs kFinishThreads/64 doubles of LDS.
+__device__ __forceinline__ double sum_partials(const double *__restrict__ part, int n, double *red)
+{
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += kFinishThreads) s += part[i];
+    return block_sum<kFinishThreads>(s, red);
 }
 
 // Element range [i0, i1) of one (XCD-renumbered) workgroup in a contiguous split of n.
@@ -124,25 +240,10 @@ __device__ __forceinline__ void elem_range(int64_t n, int64_t &i0, int64_t &i1, 
     i1 = (n * (lb + 1)) / G;
 }
 
-// Launch shapes (host).  SpMV kernels run one resident wave of workgroups (CUs x blocks that
-// fit per CU, from the occupancy calculator), each walking a contiguous tile range, so there
-// is no partial second wave; the count is a multiple of 8 for the XCD renumbering.
-inline int spmv_grid_for(int num_cu, int ntiles, int blocks_per_cu)
-{
-    int G = ntiles < num_cu * blocks_per_cu ? ntiles : num_cu * blocks_per_cu;
-    G = (G / kNumXcd) * kNumXcd;
-    return G < kNumXcd ? kNumXcd : G;
-}
-template <typename K>
-inline int resident_blocks(K kernel, int block)
-{
-    int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, block, 0) != hipSuccess || occ < 1) occ = 1;
-    return occ;
-}
-// grid for kernel `k` launched with kTileRows threads over ntiles tiles
-#define HPCCG_SPMV_GRID(k, num_cu, ntiles) \
-    ::hpccg::spmv_grid_for((num_cu), (ntiles), ::hpccg::resident_blocks((k), kTileRows))
+// Launch shapes (host).  SpMV kernels launch one wave per slice, kSellWaves slices per
+// workgroup, in slice order (the hardware dispatches workgroups in order, so the waves in
+// flight stream a compact window of the matrix); the vector kernels use a resident grid.
+inline int sell_blocks(int64_t nslices) { return (int)((nslices + kSellWaves - 1) / kSellWaves); }
 inline int vec_grid_for(int num_cu, int64_t n)
 {
     int64_t G = (n + kVecThreads - 1) / kVecThreads;
