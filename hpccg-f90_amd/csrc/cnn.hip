@@ -28,6 +28,7 @@ struct CnnNet {
     float *W = nullptr;              // FP32 master weights: W1 | W2 | W3
     float *m = nullptr, *v = nullptr;  // Adam moments
     bf16x8 *pk_f1 = nullptr, *pk_f2 = nullptr, *pk_f3 = nullptr;  // forward operands
+    bf16x8 *pk_f3f = nullptr;                                     // layer 3, fused-kernel form
     bf16x8 *pk_b3 = nullptr, *pk_b2 = nullptr;                    // backward-data operands
     int *step = nullptr;             // device training step counter
     TrainData data;
@@ -50,11 +51,15 @@ struct CnnNet {
 static CnnNet *net_of(hpccg_ctx *c) { return (CnnNet *)c->ext[1]; }
 
 // ---- weight packing into MFMA A-operands -------------------------------------------------------
-// pack_f1[l][j] = W1[co=l%16][t=k1_tap(l/16, j)]; pack_f2[s][l][j] = W2[co][ci=8((l/16)&1)+j][t=2s+(l/16)/2];
-// pack_f3 as pack_f2 with co = 0 only (W3[ci][t]); pack_b3[l][j] = W3[l%16][26 - k1_tap(l/16, j)];
+// pack_f1[l][j] = W1[co=l%16][t=k1_tap(l/16, j)];
+// pack_f2[s][l][j] = W2[co=l%16][ci=8((l/16)&1)+j][t=zk_tap(s, (l/16)/2)] (z-major K order);
+// pack_f3[s][l][j] = W3[ci][t] in row co = s/5 only (layer 3, layer-wise: row 1+dz takes the
+//   steps of plane offset dz), pack_f3f[p][l][j] = W3[ci][t = 9 co + 2p + (l/16)/2] for rows
+//   co = 0..2 (the fused kernel's one-plane form: row co is plane offset co-1);
+// pack_b3[l][j] = W3[l%16][26 - k1_tap(l/16, j)];
 // pack_b2[s][l][j] = W2[ci'][co'=l%16][26 - t] with ci' = 8((l/16)&1)+j (transposed, flipped).
-__global__ void k_pack(const float *__restrict__ W, uint16_t *f1, uint16_t *f2, uint16_t *f3, uint16_t *b3,
-                       uint16_t *b2)
+__global__ void k_pack(const float *__restrict__ W, uint16_t *f1, uint16_t *f2, uint16_t *f3, uint16_t *f3f,
+                       uint16_t *b3, uint16_t *b2)
 {
     const float *W1 = W, *W2 = W + NW1, *W3 = W + NW1 + NW2;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -65,10 +70,15 @@ __global__ void k_pack(const float *__restrict__ W, uint16_t *f1, uint16_t *f2, 
     }
     if (i < NSTEP16 * 64 * 8) {
         const int s = i / 512, l = (i / 8) % 64, j = i % 8;
-        const int co = l % 16, ci = 8 * ((l / 16) & 1) + j, t = 2 * s + (l / 16) / 2;
-        f2[i] = t < 27 ? f2bf(W2[(co * 16 + ci) * 27 + t]) : 0;
-        f3[i] = (t < 27 && co == 0) ? f2bf(W3[ci * 27 + t]) : 0;
-        b2[i] = t < 27 ? f2bf(W2[(ci * 16 + co) * 27 + (26 - t)]) : 0;
+        const int co = l % 16, ci = 8 * ((l / 16) & 1) + j, t = zk_tap(s, (l / 16) / 2);
+        f2[i] = t >= 0 ? f2bf(W2[(co * 16 + ci) * 27 + t]) : 0;
+        f3[i] = (t >= 0 && co == s / 5) ? f2bf(W3[ci * 27 + t]) : 0;
+        b2[i] = t >= 0 ? f2bf(W2[(ci * 16 + co) * 27 + (26 - t)]) : 0;
+    }
+    if (i < NSTEP3F * 64 * 8) {
+        const int p = i / 512, l = (i / 8) % 64, j = i % 8;
+        const int co = l % 16, ci = 8 * ((l / 16) & 1) + j, t9 = 2 * p + (l / 16) / 2;
+        f3f[i] = (co < 3 && t9 < 9) ? f2bf(W3[ci * 27 + 9 * co + t9]) : 0;
     }
 }
 
@@ -115,7 +125,7 @@ static int ensure_net(hpccg_ctx *c)
         hipMalloc(&N->v, NWALL * 4) != hipSuccess || hipMalloc(&N->grad, NWALL * 4) != hipSuccess ||
         hipMalloc(&N->pk_f1, 64 * 16) != hipSuccess || hipMalloc(&N->pk_b3, 64 * 16) != hipSuccess ||
         hipMalloc(&N->pk_f2, NSTEP16 * 64 * 16) != hipSuccess ||
-        hipMalloc(&N->pk_f3, NSTEP16 * 64 * 16) != hipSuccess ||
+        hipMalloc(&N->pk_f3, NSTEP16 * 64 * 16) != hipSuccess || hipMalloc(&N->pk_f3f, NSTEP3F * 64 * 16) != hipSuccess ||
         hipMalloc(&N->pk_b2, NSTEP16 * 64 * 16) != hipSuccess || hipMalloc(&N->step, sizeof(int)) != hipSuccess ||
         hipMalloc(&N->scale_in, 4096 * 4) != hipSuccess || hipMalloc(&N->scale_out, 4096 * 4) != hipSuccess) {
         set_error("cnn: allocation failed");
@@ -134,7 +144,8 @@ static int repack(hpccg_ctx *c)
 {
     CnnNet *N = net_of(c);
     hipLaunchKernelGGL(k_pack, dim3((NSTEP16 * 512 + 255) / 256), dim3(256), 0, c->stream, N->W, (uint16_t *)N->pk_f1,
-                       (uint16_t *)N->pk_f2, (uint16_t *)N->pk_f3, (uint16_t *)N->pk_b3, (uint16_t *)N->pk_b2);
+                       (uint16_t *)N->pk_f2, (uint16_t *)N->pk_f3, (uint16_t *)N->pk_f3f, (uint16_t *)N->pk_b3,
+                       (uint16_t *)N->pk_b2);
     HPCCG_HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -184,7 +195,7 @@ static void forward(hpccg_ctx *c, const float *in, const float *s_in, float *z, 
     CnnNet *N = net_of(c);
     conv<IN_F32_1, OUT_RELU_BF16>(c, in, s_in, N->pk_f1, N->h1, nullptr, nullptr, g, batch, skip);
     conv<IN_BF16_16, OUT_RELU_BF16>(c, N->h1, nullptr, N->pk_f2, N->h2, nullptr, nullptr, g, batch, skip);
-    conv<IN_BF16_16, OUT_F32_CH0>(c, N->h2, nullptr, N->pk_f3, z, nullptr, s_out, g, batch, skip);
+    conv<IN_BF16_16, OUT_F32_DZ3>(c, N->h2, nullptr, N->pk_f3, z, nullptr, s_out, g, batch, skip);
 }
 
 // inference on one field: the fused kernel (bitwise equal to forward(), no activation traffic)
@@ -193,13 +204,14 @@ static void forward_fused(hpccg_ctx *c, const float *in, const float *s_in, floa
 {
     CnnNet *N = net_of(c);
     const int tx = (g.nx + FX - 1) / FX, ty = (g.ny + FY - 1) / FY;
-    // about two workgroups per CU (the LDS rings admit two), z-chunks of at least 8 planes
-    const int want = std::max(1, (2 * c->info.num_cu + tx * ty - 1) / (tx * ty));
+    // about one workgroup per CU (registers admit one 8-wave workgroup), z-chunks of at least
+    // 8 planes (each chunk re-reads 6 planes of halo)
+    const int want = std::max(1, (c->info.num_cu + tx * ty - 1) / (tx * ty));
     const int nzc = std::max(1, std::min(want, (g.nz + 7) / 8));
     const int zchunk = (g.nz + nzc - 1) / nzc;
     const int chunks = (g.nz + zchunk - 1) / zchunk;
-    hipLaunchKernelGGL(k_fused_fwd, dim3(tx * ty * chunks), dim3(256), 0, c->stream, in, s_in, N->pk_f1, N->pk_f2,
-                       N->pk_f3, z, s_out, g.nx, g.ny, g.nz, tx, ty, zchunk, skip);
+    hipLaunchKernelGGL(k_fused_fwd, dim3(tx * ty * chunks), dim3(FTHREADS), 0, c->stream, in, s_in, N->pk_f1,
+                       N->pk_f2, N->pk_f3f, z, s_out, g.nx, g.ny, g.nz, tx, ty, zchunk, skip);
 }
 
 // backward from dz (BF16 [b][v]) and the stored forward activations: grad = dL/dW
@@ -641,7 +653,7 @@ void free_learned(hpccg_ctx *c)
     if (CnnNet *N = net_of(c)) {
         if (N->train_graph) hipGraphExecDestroy(N->train_graph);
         for (void *p : {(void *)N->W, (void *)N->m, (void *)N->v, (void *)N->pk_f1, (void *)N->pk_f2,
-                        (void *)N->pk_f3, (void *)N->pk_b3, (void *)N->pk_b2, (void *)N->step, (void *)N->h1,
+                        (void *)N->pk_f3, (void *)N->pk_f3f, (void *)N->pk_b3, (void *)N->pk_b2, (void *)N->step, (void *)N->h1,
                         (void *)N->h2, (void *)N->dh2, (void *)N->dh1, (void *)N->dz, (void *)N->in, (void *)N->z,
                         (void *)N->tgt, (void *)N->wbuf, (void *)N->scale_in, (void *)N->scale_out,
                         (void *)N->partials, (void *)N->grad, (void *)N->loss, (void *)N->lpart,
@@ -938,8 +950,8 @@ int hpccg_cnn_train(hpccg_ctx *c, int32_t steps, int32_t batch, float lr, uint64
     hipLaunchKernelGGL(k_adam, dim3((NWALL + 255) / 256), dim3(256), 0, c->stream, N->W, N->m, N->v, N->grad, NWALL,
                        lr, steps, N->step);
     hipLaunchKernelGGL(k_pack, dim3((NSTEP16 * 512 + 255) / 256), dim3(256), 0, c->stream, N->W,
-                       (uint16_t *)N->pk_f1, (uint16_t *)N->pk_f2, (uint16_t *)N->pk_f3, (uint16_t *)N->pk_b3,
-                       (uint16_t *)N->pk_b2);
+                       (uint16_t *)N->pk_f1, (uint16_t *)N->pk_f2, (uint16_t *)N->pk_f3, (uint16_t *)N->pk_f3f,
+                       (uint16_t *)N->pk_b3, (uint16_t *)N->pk_b2);
     hipLaunchKernelGGL(k_step_inc, dim3(1), dim3(64), 0, c->stream, N->step);
     const hipError_t ec = hipStreamEndCapture(c->stream, &gr);
     HPCCG_HIP_TRY(ec);

@@ -7,8 +7,14 @@
 // ([batch][voxel][16], 32 bytes per voxel), accumulation is FP32 (MFMA).
 //
 // Implicit GEMM on v_mfma_f32_16x16x32_bf16 (D[16x16] += A[16x32] * B[32x16]):
-//   M = output channels (16), N = 16 voxels, K = 2 taps x 16 input channels per MFMA
-//   (14 MFMAs for the 27 taps, the 28th tap zero), or K = 27(+5) taps for a 1-channel input.
+//   M = output channels (16), N = 16 voxels, K = 2 taps x 16 input channels per MFMA, or
+//   K = 27(+5) taps for a 1-channel input.  16-channel K order ("z-major"): 15 K-steps
+//   s = 5 (dz+1) + p, each pairing the in-plane taps 2p and 2p+1 of the 3 x 3 taps of plane
+//   offset dz (the tenth in-plane slot is zero), so that the five operands of one input plane
+//   serve all three output planes it touches.
+//   The one-channel output of layer 3 is formed as three partial sums, one per plane offset
+//   dz, in MFMA rows 0..2 (row 1+dz takes only the steps of offset dz), then added in FP32 in
+//   the order ((P(-1) + P(0)) + P(+1)).
 //   A = weights, pre-packed per lane and held in registers for the whole kernel.
 //   B = activations from an LDS halo tile: each lane reads the 8 channels it needs as one
 //   16-byte LDS load.
@@ -20,6 +26,8 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 
 #include "common.hpp"
 
@@ -35,18 +43,34 @@ constexpr int TX = 8, TY = 8, TZ = 4;
 constexpr int HX = TX + 2, HY = TY + 2, HZ = TZ + 2;
 constexpr int HV = HX * HY * HZ;  // 600 halo voxels
 constexpr int TV = TX * TY * TZ;  // 256 voxels per tile
-constexpr int NSTEP16 = 14;       // MFMA K-steps for 16 input channels x 27 taps
+constexpr int NSTEP16 = 15;       // MFMA K-steps for 16 input channels x 27 taps (z-major)
+constexpr int NSTEP3F = 5;        // K-steps of one input plane for the three-row layer-3 form
 constexpr int NW1 = 16 * 27, NW2 = 16 * 16 * 27, NW3 = 16 * 27;
 constexpr int NWALL = NW1 + NW2 + NW3;  // 7776 parameters
 
-// float -> bfloat16, round to nearest even (inf/nan pass through)
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+// two floats -> two bfloat16 in one word (low half = a), round to nearest even: one
+// v_cvt_pk_bf16_f32 on gfx950
+__device__ inline uint32_t pack_bf16(float a, float b)
+{
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{a, b}, bf16x2));
+}
+
+// float -> bfloat16, round to nearest even (inf/nan pass through, NaN quieted); the device
+// form is the hardware conversion, the host form the same rounding on the bit pattern
 __host__ __device__ inline uint16_t f2bf(float f)
 {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return (uint16_t)(pack_bf16(f, 0.f) & 0xffffu);
+#else
     uint32_t u;
     __builtin_memcpy(&u, &f, 4);
     if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
     u += 0x7fffu + ((u >> 16) & 1u);
     return (uint16_t)(u >> 16);
+#endif
 }
 __host__ __device__ inline float bf2f(uint16_t h)
 {
@@ -72,9 +96,26 @@ __host__ __device__ inline int k1_tap(int c, int j)
     return -1;
 }
 
+// Tap of K-step s (z-major order) for the lanes of operand row half `hi` (0: taps 2p, 1:
+// taps 2p+1); -1 for the zero slot.
+__host__ __device__ inline int zk_tap(int s, int hi)
+{
+    const int dz1 = s / 5, t9 = 2 * (s % 5) + hi;
+    return t9 < 9 ? 9 * dz1 + t9 : -1;
+}
+
 __device__ inline f32x4 mfma16(bf16x8 a, bf16x8 b, f32x4 c)
 {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// ReLU then BF16 of the four accumulator rows of a lane, packed as 4 x 16 bits
+__device__ inline u32x2 relu_bf16x4(f32x4 a)
+{
+    u32x2 pk;
+    pk[0] = pack_bf16(__builtin_fmaxf(a[0], 0.f), __builtin_fmaxf(a[1], 0.f));
+    pk[1] = pack_bf16(__builtin_fmaxf(a[2], 0.f), __builtin_fmaxf(a[3], 0.f));
+    return pk;
 }
 
 struct Grid3 {
@@ -92,9 +133,10 @@ inline Grid3 make_grid(int nx, int ny, int nz)
 // Input of a convolution: BF16 channel-last activations, or (layer 1) an FP32 field multiplied
 // by a per-sample scale and rounded to BF16 while it is staged.
 enum InKind { IN_BF16_16 = 0, IN_BF16_1 = 1, IN_F32_1 = 2 };
-// Epilogue: ReLU -> BF16 [v][16]; FP32 channel 0 times a per-sample scale; masked BF16 (the
-// backward pass through a ReLU whose forward output is `mask`).
-enum OutKind { OUT_RELU_BF16 = 0, OUT_F32_CH0 = 1, OUT_MASKED_BF16 = 2 };
+// Epilogue: ReLU -> BF16 [v][16]; FP32 ((row0 + row1) + row2) times a per-sample scale (the
+// one-channel layer 3 from its three plane-offset partials); masked BF16 (the backward pass
+// through a ReLU whose forward output is `mask`).
+enum OutKind { OUT_RELU_BF16 = 0, OUT_F32_DZ3 = 1, OUT_MASKED_BF16 = 2 };
 
 // Stage the 10x10x6 halo of tile (ox,oy,oz) of sample b into LDS (channel-last, 16 or 1 ch).
 template <int IK>
@@ -129,8 +171,8 @@ __device__ inline void stage_halo(const void *in, float scale, uint16_t *sm, con
     }
 }
 
-// One 3x3x3 convolution layer with 16 output channels (layer 3 packs its single output channel
-// as channel 0 of 16).  Grid: one workgroup per (tile, sample).
+// One 3x3x3 convolution layer with 16 output channels (layer 3: its single output channel as
+// three plane-offset partials in rows 0..2).  Grid: one workgroup per (tile, sample).
 template <int IK, int OK>
 __global__ __launch_bounds__(256) void k_conv(const void *__restrict__ in, const float *__restrict__ in_scale,
                                               const bf16x8 *__restrict__ wpack, void *__restrict__ out,
@@ -163,9 +205,9 @@ __global__ __launch_bounds__(256) void k_conv(const void *__restrict__ in, const
         if (CIN == 16) {
 #pragma unroll
             for (int s = 0; s < NSTEP; ++s) {
-                const int t = 2 * s + c / 2;
+                const int t = zk_tap(s, c / 2);
                 u32x4 raw = {0u, 0u, 0u, 0u};
-                if (t < 27) {
+                if (t >= 0) {
                     const int hv = (z + tap_dz(t) + 1) * (HY * HX) + (y + tap_dy(t) + 1) * HX + (x + tap_dx(t) + 1);
                     raw = *(const u32x4 *)(sm + hv * 16 + (c & 1) * 8);
                 }
@@ -193,23 +235,22 @@ __global__ __launch_bounds__(256) void k_conv(const void *__restrict__ in, const
         const int gx = ox + x, gy = oy + y, gz = oz + z;
         if (gx < g.nx && gy < g.ny && gz < g.nz) {
             const int64_t gv = (int64_t)b * N + ((int64_t)gz * g.ny + gy) * g.nx + gx;
-            if (OK == OUT_F32_CH0) {
-                if (c == 0) ((float *)out)[gv] = acc[0] * (out_scale ? out_scale[b] : 1.0f);
+            if (OK == OUT_F32_DZ3) {
+                if (c == 0) ((float *)out)[gv] = ((acc[0] + acc[1]) + acc[2]) * (out_scale ? out_scale[b] : 1.0f);
             } else {
-                uint16_t o[4];
+                u32x2 pk;
                 if (OK == OUT_RELU_BF16) {
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) o[i] = f2bf(acc[i] > 0.f ? acc[i] : 0.f);
+                    pk = relu_bf16x4(acc);
                 } else {
                     const u32x2 mm = *(const u32x2 *)(mask + gv * 16 + 4 * c);
                     const uint16_t m4[4] = {(uint16_t)(mm[0] & 0xffff), (uint16_t)(mm[0] >> 16),
                                             (uint16_t)(mm[1] & 0xffff), (uint16_t)(mm[1] >> 16)};
+                    uint16_t o[4];
 #pragma unroll
                     for (int i = 0; i < 4; ++i) o[i] = bf2f(m4[i]) > 0.f ? f2bf(acc[i]) : (uint16_t)0;
+                    pk[0] = o[0] | ((uint32_t)o[1] << 16);
+                    pk[1] = o[2] | ((uint32_t)o[3] << 16);
                 }
-                u32x2 pk;
-                pk[0] = o[0] | ((uint32_t)o[1] << 16);
-                pk[1] = o[2] | ((uint32_t)o[3] << 16);
                 *(u32x2 *)((uint16_t *)out + gv * 16 + 4 * c) = pk;
             }
         }
@@ -299,37 +340,50 @@ __global__ __launch_bounds__(256) void k_wgrad(const uint16_t *__restrict__ dout
 }
 
 // ---- fused inference: the three layers in one kernel -------------------------------------------
-// A workgroup owns a 16 x 16 (x, y) column of output voxels and a range of z-planes, and marches
-// along z keeping three planes of each layer's input in LDS rings: step i loads input plane i
-// (rounded to BF16), computes h1 plane i-1, h2 plane i-2 and the output plane i-3.  h1 is
-// computed on a 20 x 20 and h2 on an 18 x 18 footprint (the halos the next layer needs), so the
-// activations never leave LDS.  The MFMA operands, their K order and the epilogues are those of
-// k_conv, so the output is bitwise the three-kernel forward pass's.
+// A workgroup owns a 16 x 16 (x, y) column of output voxels and a range of z-planes and marches
+// along z.  Step i stages input plane i (rounded to BF16) into a three-plane window and then
+//   layer 1: computes h1 plane i-1 from the window (one MFMA per 16 voxels, K = 27 taps),
+//   layer 2: reads h1 plane i-1 ONCE -- five operands per 16 voxels, one per in-plane tap pair
+//            -- and adds it to the three h2 planes it touches (15 MFMAs), h2 plane i-2 being
+//            complete afterwards;
+//   layer 3: reads h2 plane i-2 once the same way (five MFMAs, the three plane-offset partials
+//            in rows 0..2) and completes output plane i-3.
+// The h2 and output partial sums of unfinished planes stay in registers, rotating through three
+// slots (the loop body is unrolled three times so that the slots are compile-time registers),
+// so only one plane of h1 and one of h2 are kept in LDS, and every LDS operand feeds three
+// MFMAs.  h1 is computed on a 20 x 20 and h2 on an 18 x 18 footprint (the halos the next
+// layer needs).  Each accumulator receives its K-steps in the order of the layer-wise k_conv
+// (z-major), so the output is bitwise the three-kernel forward pass's.
 constexpr int FX = 16, FY = 16;             // output footprint per workgroup
 constexpr int F2X = FX + 2, F2Y = FY + 2;   // h2 footprint (324 voxels, 21 groups of 16)
 constexpr int F1X = FX + 4, F1Y = FY + 4;   // h1 footprint (400 voxels, 25 groups)
 constexpr int F0X = FX + 6, F0Y = FY + 6;   // input footprint (484 voxels)
 constexpr int F2V = F2X * F2Y, F1V = F1X * F1Y, F0V = F0X * F0Y;
 constexpr int F2G = (F2V + 15) / 16, F1G = F1V / 16, F3G = FX * FY / 16;
+constexpr int FWAVES = 8, FTHREADS = 64 * FWAVES;  // waves per fused workgroup
+constexpr int G2W = (F2G + FWAVES - 1) / FWAVES, G3W = F3G / FWAVES;  // layer-2 / -3 voxel groups per wave
 
-__device__ inline int ring3(int p) { return ((p % 3) + 3) % 3; }
+template <int K>
+using Slot = std::integral_constant<int, K>;
 
 // Layer 1 reads its taps as three x-triplets per lane (the input window keeps its planes in
-// order, so the offsets are fixed per lane); layers 2 and 3 resolve both candidate taps of each K-step at compile time
-// (a lane's row of the B operand picks one), load all 14 operands of a voxel group before its
-// MFMA chain, and keep h1/h2 in three-plane rings.  Two workgroups fit a CU (78 KB of LDS each).
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_fused_fwd(const float *__restrict__ in, const float *__restrict__ in_scale,
-                                               const bf16x8 *__restrict__ pk1, const bf16x8 *__restrict__ pk2,
-                                               const bf16x8 *__restrict__ pk3, float *__restrict__ out,
-                                               const float *__restrict__ out_scale, int nx, int ny, int nz,
-                                               int tiles_x, int tiles_y, int zchunk, const int *__restrict__ skip)
+// order, so the offsets are fixed per lane).  Every LDS address and grid-boundary test that
+// does not depend on the plane is formed once, before the march: a lane's voxel groups are
+// the same on every plane.
+__global__ __launch_bounds__(FTHREADS) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_fused_fwd(
+    const float *__restrict__ in, const float *__restrict__ in_scale, const bf16x8 *__restrict__ pk1,
+    const bf16x8 *__restrict__ pk2, const bf16x8 *__restrict__ pk3f, float *__restrict__ out,
+    const float *__restrict__ out_scale, int nx, int ny, int nz, int tiles_x, int tiles_y, int zchunk,
+    const int *__restrict__ skip)
 {
     if (skip && *skip) return;
     // input window, planes i-2, i-1, i, stored as x-triplets: T[plane][row][x] holds the input at
     // window columns x, x+1, x+2 (one 8-byte read per tap line of layer 1)
     __shared__ __attribute__((aligned(16))) u32x2 T[3][F0Y][F1X];
-    __shared__ __attribute__((aligned(16))) uint16_t H1[3][F1V * 16];
-    __shared__ __attribute__((aligned(16))) uint16_t H2[3][F2V * 16];
+    // one plane of h1 / h2, each followed by 8 zero channels (the operand of the empty tenth
+    // tap slot)
+    __shared__ __attribute__((aligned(16))) uint16_t H1[F1V * 16 + 8];
+    __shared__ __attribute__((aligned(16))) uint16_t H2[F2V * 16 + 8];
     const int txy = blockIdx.x % (tiles_x * tiles_y), zc = blockIdx.x / (tiles_x * tiles_y);
     const int ox = (txy % tiles_x) * FX, oy = (txy / tiles_x) * FY;
     const int z0 = zc * zchunk, z1 = min(z0 + zchunk, nz);
@@ -337,155 +391,185 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
     const float oscale = out_scale ? out_scale[0] : 1.0f;
     const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
     const int c = lane / 16, n = lane % 16;
-    const bool hi = c >= 2;       // second tap of each K-step pair
-    const int half = (c & 1) * 8; // channel half
-    bf16x8 A1 = pk1[lane];
-    bf16x8 A2[NSTEP16], A3[NSTEP16];
+    const bool hi = c >= 2;        // operand rows of the second tap of each pair
+    const int half = (c & 1) * 8;  // channel half
+    const bf16x8 A1 = pk1[lane];
+    bf16x8 A2[NSTEP16], A3[NSTEP3F];
 #pragma unroll
-    for (int s = 0; s < NSTEP16; ++s) { A2[s] = pk2[s * 64 + lane]; A3[s] = pk3[s * 64 + lane]; }
+    for (int s = 0; s < NSTEP16; ++s) A2[s] = pk2[s * 64 + lane];
+#pragma unroll
+    for (int s = 0; s < NSTEP3F; ++s) A3[s] = pk3f[s * 64 + lane];
+    if (threadIdx.x < 8) {
+        H1[F1V * 16 + threadIdx.x] = 0;
+        H2[F2V * 16 + threadIdx.x] = 0;
+    }
     const int64_t plane = (int64_t)nx * ny;
-    // layer-1 tap lines of this lane's B-operand row (k1_tap order): lines 2c, 2c+1 and line 8
-    const int lA = 2 * c, lB = 2 * c + 1;
-    const int offA = ((lA / 3) * F0Y + lA % 3) * F1X, offB = ((lB / 3) * F0Y + lB % 3) * F1X;
-    const int offC = (2 * F0Y + 2) * F1X;
-    const u32x2 *T0 = &T[0][0][0];
-    // triplet e of a plane: window row e / F1X, columns e % F1X .. +2 (440 per plane, two per thread)
-    auto load_triplet = [&](int ip, int e) -> u32x2 {
+
+    // ---- plane-independent per-lane facts ----
+    // layer 1: group k is g = w + 8k, voxel v = 16 g + n of the 20 x 20 footprint (T row = v)
+    constexpr int G1W = (F1G + FWAVES - 1) / FWAVES;
+    const int v1 = 16 * w + n;
+    const int lA = 2 * c, lB = 2 * c + 1;  // layer-1 tap lines of this lane's operand row
+    const u32x2 *TA = &T[0][0][0] + ((lA / 3) * F0Y + lA % 3) * F1X + v1;
+    const u32x2 *TB = &T[0][0][0] + ((lB / 3) * F0Y + lB % 3) * F1X + v1;
+    const u32x2 *TC = &T[0][0][0] + (2 * F0Y + 2) * F1X + v1;
+    uint16_t *H1w = H1 + v1 * 16 + 4 * c;
+    unsigned xy1 = 0;  // bit k: group k exists and its voxel lies inside the grid (x, y)
+#pragma unroll
+    for (int k = 0; k < G1W; ++k) {
+        const int g = w + FWAVES * k, v = v1 + 16 * FWAVES * k;
+        const int gx = ox - 2 + v % F1X, gy = oy - 2 + v / F1X;
+        if (g < F1G && gx >= 0 && gx < nx && gy >= 0 && gy < ny) xy1 |= 1u << k;
+    }
+    // layers 2 and 3 read operand pair p at base + off(2p) (+ the second tap for hi lanes):
+    // off(t9) = (t9 / 3) RX + t9 % 3 voxels; the hi lanes add 1 voxel (pairs 0, 2, 3), RX - 2
+    // (pair 1), or read the zero channels (pair 4)
+    const int hi16 = hi ? 16 : 0;
+    int b2[G2W];       // layer-2 operand base of group k (hi offset included), in uint16 units
+    unsigned xy2 = 0;  // bit k: h2 voxel of group k exists and lies inside the grid (x, y)
+#pragma unroll
+    for (int k = 0; k < G2W; ++k) {
+        const int g = w + FWAVES * k, v = 16 * g + n, vc = min(v, F2V - 1), hx = vc % F2X, hy = vc / F2X;
+        b2[k] = (hy * F1X + hx) * 16 + half + hi16;
+        const int gx = ox - 1 + hx, gy = oy - 1 + hy;
+        if (g < F2G && v < F2V && gx >= 0 && gx < nx && gy >= 0 && gy < ny) xy2 |= 1u << k;
+    }
+    uint16_t *H2w = H2 + v1 * 16 + 4 * c;
+    const int b3 = ((w * F2X) + n) * 16 + half + hi16;  // layer-3 operand base of group 0
+    const int64_t ob = (int64_t)(oy + w) * nx + ox + n;  // output offset of group 0 in a plane
+    unsigned xy3 = 0;
+#pragma unroll
+    for (int k = 0; k < G3W; ++k)
+        if (c == 0 && ox + n < nx && oy + w + FWAVES * k < ny) xy3 |= 1u << k;
+
+    // five B operands of a voxel group whose operand base (hi offset included) is `base`, in a
+    // one-plane buffer H of row length RX followed by zero channels at ZOFF
+    auto load_pairs = [&](const uint16_t *H, auto RXc, int ZOFF, int base, u32x4 *raw) {
+        constexpr int RX = decltype(RXc)::value;
+        const uint16_t *pb = H + base;
+        raw[0] = *(const u32x4 *)(pb);
+        raw[1] = *(const u32x4 *)(pb + (hi ? (RX - 3) * 16 : 0) + 2 * 16);
+        raw[2] = *(const u32x4 *)(pb + (RX + 1) * 16);
+        raw[3] = *(const u32x4 *)(pb + (2 * RX) * 16);
+        raw[4] = *(const u32x4 *)(hi ? H + ZOFF : pb + (2 * RX + 2) * 16);
+    };
+    // triplet e of a plane: window row e / F1X, columns e % F1X .. +2 (440 per plane, one per thread)
+    const bool trip = threadIdx.x < F0Y * F1X;
+    const int tgy = oy - 3 + (int)threadIdx.x / F1X, tgx0 = ox - 3 + (int)threadIdx.x % F1X;
+    const bool trow = trip && tgy >= 0 && tgy < ny;
+    auto load_triplet = [&](int ip) -> u32x2 {
         u32x2 t3;
         t3[0] = t3[1] = 0u;
-        if (e >= F0Y * F1X || ip < 0 || ip >= nz) return t3;
-        const int gy = oy - 3 + e / F1X, gx0 = ox - 3 + e % F1X;
-        if (gy < 0 || gy >= ny) return t3;
-        const float *row = in + ip * plane + (int64_t)gy * nx;
-        uint16_t u[3];
+        if (!trow || ip < 0 || ip >= nz) return t3;
+        const float *row = in + ip * plane + (int64_t)tgy * nx;
+        float u[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            const int gx = gx0 + k;
-            u[k] = (gx >= 0 && gx < nx) ? f2bf(row[gx] * scale) : (uint16_t)0;
+            const int gx = tgx0 + k;
+            u[k] = (gx >= 0 && gx < nx) ? row[gx] * scale : 0.f;
         }
-        t3[0] = u[0] | ((uint32_t)u[1] << 16);
-        t3[1] = u[2];
+        t3[0] = pack_bf16(u[0], u[1]);
+        t3[1] = pack_bf16(u[2], 0.f);
         return t3;
     };
-    u32x2 pre[2];
+
+    f32x4 acc2[3][G2W];
+    float acc3[3][G3W];
 #pragma unroll
-    for (int e = 0; e < 2; ++e) pre[e] = load_triplet(z0 - 3, threadIdx.x + 256 * e);
-    int r0 = ring3(z0 - 3);  // ring slot of plane z0-3; slots advance by one per step
-    for (int i = z0 - 3; i <= z1 + 2; ++i) {
-        // three ring slots: plane i -> si, i-1 -> sm1, i-2 -> sm2 (and i-3 -> si, i-4 -> sm1)
-        const int si = r0, sm1 = r0 == 0 ? 2 : r0 - 1, sm2 = sm1 == 0 ? 2 : sm1 - 1;
-        r0 = r0 == 2 ? 0 : r0 + 1;
-        {
-            // shift the window (each thread owns the same triplets of all three planes) and
-            // store plane i, prefetched during the previous step; then prefetch plane i + 1
+    for (int r = 0; r < 3; ++r) {
 #pragma unroll
-            for (int e = 0; e < 2; ++e) {
-                const int v = threadIdx.x + 256 * e;
-                if (v < F0Y * F1X) {
-                    u32x2 *Tv = &T[0][0][0] + v;
-                    const u32x2 a1 = Tv[F0Y * F1X], a2 = Tv[2 * F0Y * F1X];
-                    Tv[0] = a1;
-                    Tv[F0Y * F1X] = a2;
-                    Tv[2 * F0Y * F1X] = pre[e];
-                }
-            }
+        for (int k = 0; k < G2W; ++k) acc2[r][k] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int e = 0; e < 2; ++e) pre[e] = load_triplet(i + 1, threadIdx.x + 256 * e);
+        for (int k = 0; k < G3W; ++k) acc3[r][k] = 0.f;
+    }
+    u32x2 pre = load_triplet(z0 - 3);
+
+    // one z-step; slot R holds the partial sums of the planes the step starts (h2 plane i,
+    // output plane i-1); slot R+2 those it continues; slot R+1 those it completes
+    auto step = [&](auto R, int i) {
+        constexpr int NXT = decltype(R)::value, CUR = (NXT + 2) % 3, PRV = (NXT + 1) % 3;
+        // shift the window (each thread owns the same triplet of all three planes) and store
+        // plane i, prefetched during the previous step; then prefetch plane i + 1
+        if (trip) {
+            u32x2 *Tv = &T[0][0][0] + threadIdx.x;
+            const u32x2 a1 = Tv[F0Y * F1X], a2 = Tv[2 * F0Y * F1X];
+            Tv[0] = a1;
+            Tv[F0Y * F1X] = a2;
+            Tv[2 * F0Y * F1X] = pre;
         }
+        pre = load_triplet(i + 1);
         __syncthreads();
-        const int q = i - 1;  // h1 plane q uses input planes q-1, q, q+1 = window planes 0, 1, 2
+        const int q = i - 1;  // h1 plane q from input planes q-1, q, q+1 (window planes 0, 1, 2)
         if (q >= z0 - 2) {
-            uint16_t *Hq = H1[sm1];
             const bool zin = q >= 0 && q < nz;
-            for (int g = w; g < F1G; g += 4) {
-                const int v = 16 * g + n, hx = v % F1X, hy = v / F1X;
-                const int gb = hy * F1X + hx;
-                const u32x2 ta = T0[offA + gb], tb = T0[offB + gb], tc = T0[offC + gb];
-                u32x4 raw;
-                raw[0] = ta[0];
-                raw[1] = (ta[1] & 0xffffu) | (tb[0] << 16);
-                raw[2] = (tb[0] >> 16) | (tb[1] << 16);
-                raw[3] = c == 0 ? tc[0] : (c == 1 ? (tc[1] & 0xffffu) : 0u);
-                const f32x4 acc = mfma16(A1, __builtin_bit_cast(bf16x8, raw), f32x4{0.f, 0.f, 0.f, 0.f});
-                const int gx = ox - 2 + hx, gy = oy - 2 + hy;
-                const bool in_grid = zin && gx >= 0 && gx < nx && gy >= 0 && gy < ny;
-                u32x2 pk; pk[0] = pk[1] = 0u;
-                if (in_grid) {
-                    pk[0] = f2bf(acc[0] > 0.f ? acc[0] : 0.f) | ((uint32_t)f2bf(acc[1] > 0.f ? acc[1] : 0.f) << 16);
-                    pk[1] = f2bf(acc[2] > 0.f ? acc[2] : 0.f) | ((uint32_t)f2bf(acc[3] > 0.f ? acc[3] : 0.f) << 16);
+#pragma unroll
+            for (int k = 0; k < G1W; ++k) {
+                if (w + FWAVES * k < F1G) {
+                    const int o16 = 16 * FWAVES * k;
+                    const u32x2 ta = TA[o16], tb = TB[o16], tc = TC[o16];
+                    u32x4 raw;
+                    raw[0] = ta[0];
+                    raw[1] = (ta[1] & 0xffffu) | (tb[0] << 16);
+                    raw[2] = (tb[0] >> 16) | (tb[1] << 16);
+                    raw[3] = c == 0 ? tc[0] : (c == 1 ? (tc[1] & 0xffffu) : 0u);
+                    const f32x4 acc = mfma16(A1, __builtin_bit_cast(bf16x8, raw), f32x4{0.f, 0.f, 0.f, 0.f});
+                    u32x2 pk = relu_bf16x4(acc);
+                    if (!(zin && ((xy1 >> k) & 1u))) pk[0] = pk[1] = 0u;
+                    *(u32x2 *)(H1w + o16 * 16) = pk;
                 }
-                *(u32x2 *)(Hq + v * 16 + 4 * c) = pk;
             }
         }
         __syncthreads();
-        const int p = i - 2;  // h2 plane p uses H1 planes p-1 (slot of i-3 == si), p (sm2), p+1 (sm1)
-        if (p >= z0 - 1) {
-            const uint16_t *Hp3[3] = {H1[si], H1[sm2], H1[sm1]};
-            uint16_t *Hp = H2[sm2];
-            const bool zin = p >= 0 && p < nz;
-            for (int g = w; g < F2G; g += 4) {
-                const int v = 16 * g + n, vc = min(v, F2V - 1), hx = vc % F2X, hy = vc / F2X;
-                const int gb = (hy * F1X + hx) * 16 + half;
-                f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-                u32x4 raw[NSTEP16];
+        // layer 2: h1 plane q feeds h2 planes q+1 (dz = -1), q (dz = 0) and q-1 (dz = +1, done)
+        if (q >= z0 - 2) {
+            const int pd = q - 1;
+            const bool keep = pd >= z0 - 1, zin = pd >= 0 && pd < nz;
 #pragma unroll
-                for (int s = 0; s < NSTEP16; ++s) {
-                    const int tA = 2 * s, tB = 2 * s + 1;
-                    const uint16_t *pa = Hp3[tap_dz(tA) + 1] + ((tap_dy(tA) + 1) * F1X + tap_dx(tA) + 1) * 16;
-                    if (tB < 27) {
-                        const uint16_t *pb = Hp3[tap_dz(tB) + 1] + ((tap_dy(tB) + 1) * F1X + tap_dx(tB) + 1) * 16;
-                        raw[s] = *(const u32x4 *)((hi ? pb : pa) + gb);
-                    } else {
-                        raw[s] = *(const u32x4 *)(pa + gb);
-                        if (hi) raw[s] = u32x4{0u, 0u, 0u, 0u};
+            for (int k = 0; k < G2W; ++k) {
+                if (w + FWAVES * k < F2G) {
+                    u32x4 raw[NSTEP3F];
+                    load_pairs(H1, std::integral_constant<int, F1X>{}, F1V * 16, b2[k], raw);
+                    f32x4 an = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                    for (int p = 0; p < NSTEP3F; ++p) {
+                        const bf16x8 bop = __builtin_bit_cast(bf16x8, raw[p]);
+                        an = mfma16(A2[p], bop, an);
+                        acc2[CUR][k] = mfma16(A2[5 + p], bop, acc2[CUR][k]);
+                        acc2[PRV][k] = mfma16(A2[10 + p], bop, acc2[PRV][k]);
                     }
+                    u32x2 pk = relu_bf16x4(acc2[PRV][k]);
+                    acc2[NXT][k] = an;
+                    if (!(zin && ((xy2 >> k) & 1u))) pk[0] = pk[1] = 0u;
+                    if (keep && 16 * (w + FWAVES * k) + n < F2V) *(u32x2 *)(H2w + 16 * FWAVES * k * 16) = pk;
                 }
-#pragma unroll
-                for (int s = 0; s < NSTEP16; ++s) acc = mfma16(A2[s], __builtin_bit_cast(bf16x8, raw[s]), acc);
-                // schedule: the 14 LDS reads first, then the 14 MFMAs (the default schedule
-                // issues each read just before its MFMA and waits on it)
-                __builtin_amdgcn_sched_group_barrier(0x100, 14, 0);
-                __builtin_amdgcn_sched_group_barrier(0x008, 14, 0);
-                const int gx = ox - 1 + hx, gy = oy - 1 + hy;
-                const bool in_grid = zin && gx >= 0 && gx < nx && gy >= 0 && gy < ny;
-                u32x2 pk; pk[0] = pk[1] = 0u;
-                if (in_grid) {
-                    pk[0] = f2bf(acc[0] > 0.f ? acc[0] : 0.f) | ((uint32_t)f2bf(acc[1] > 0.f ? acc[1] : 0.f) << 16);
-                    pk[1] = f2bf(acc[2] > 0.f ? acc[2] : 0.f) | ((uint32_t)f2bf(acc[3] > 0.f ? acc[3] : 0.f) << 16);
-                }
-                if (v < F2V) *(u32x2 *)(Hp + v * 16 + 4 * c) = pk;
             }
         }
         __syncthreads();
-        const int o = i - 3;  // output plane o uses H2 planes o-1 (slot of i-4 == sm1), o (si), o+1 (sm2)
-        if (o >= z0 && o < z1) {
-            const uint16_t *Ho3[3] = {H2[sm1], H2[si], H2[sm2]};
-            for (int g = w; g < F3G; g += 4) {
-                const int gb = (g * F2X + n) * 16 + half;
-                f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-                u32x4 raw[NSTEP16];
+        // layer 3: h2 plane pp feeds output planes pp+1 (row 0), pp (row 1), pp-1 (row 2, done)
+        const int pp = i - 2;
+        if (pp >= z0 - 1) {
+            const int o = pp - 1;
+            const bool store = o >= z0 && o < z1;
 #pragma unroll
-                for (int s = 0; s < NSTEP16; ++s) {
-                    const int tA = 2 * s, tB = 2 * s + 1;
-                    const uint16_t *pa = Ho3[tap_dz(tA) + 1] + ((tap_dy(tA) + 1) * F2X + tap_dx(tA) + 1) * 16;
-                    if (tB < 27) {
-                        const uint16_t *pb = Ho3[tap_dz(tB) + 1] + ((tap_dy(tB) + 1) * F2X + tap_dx(tB) + 1) * 16;
-                        raw[s] = *(const u32x4 *)((hi ? pb : pa) + gb);
-                    } else {
-                        raw[s] = *(const u32x4 *)(pa + gb);
-                        if (hi) raw[s] = u32x4{0u, 0u, 0u, 0u};
-                    }
-                }
+            for (int k = 0; k < G3W; ++k) {
+                u32x4 raw[NSTEP3F];
+                load_pairs(H2, std::integral_constant<int, F2X>{}, F2V * 16, b3 + FWAVES * k * F2X * 16, raw);
+                f32x4 d = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                for (int s = 0; s < NSTEP16; ++s) acc = mfma16(A3[s], __builtin_bit_cast(bf16x8, raw[s]), acc);
-                // schedule: the 14 LDS reads first, then the 14 MFMAs (the default schedule
-                // issues each read just before its MFMA and waits on it)
-                __builtin_amdgcn_sched_group_barrier(0x100, 14, 0);
-                __builtin_amdgcn_sched_group_barrier(0x008, 14, 0);
-                const int gx = ox + n, gy = oy + g;
-                if (c == 0 && gx < nx && gy < ny) out[o * plane + (int64_t)gy * nx + gx] = acc[0] * oscale;
+                for (int p = 0; p < NSTEP3F; ++p) d = mfma16(A3[p], __builtin_bit_cast(bf16x8, raw[p]), d);
+                acc3[NXT][k] = d[0];
+                acc3[CUR][k] = acc3[CUR][k] + d[1];
+                acc3[PRV][k] = acc3[PRV][k] + d[2];
+                if (store && ((xy3 >> k) & 1u))
+                    out[o * plane + ob + (int64_t)FWAVES * k * nx] = acc3[PRV][k] * oscale;
             }
         }
+    };
+    const int last = z1 + 2;
+    for (int i = z0 - 3; i <= last; i += 3) {
+        step(Slot<0>{}, i);
+        if (i + 1 <= last) step(Slot<1>{}, i + 1);
+        if (i + 2 <= last) step(Slot<2>{}, i + 2);
     }
 }
 
