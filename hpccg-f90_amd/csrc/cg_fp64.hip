@@ -108,13 +108,11 @@ __global__ __launch_bounds__(kFinishThreads) void k_cg_pap_finish(const double *
     if (threadIdx.x == 0) cg_after_pap(st, rr_hist, k, pap);
 }
 
-// Prepare: r = 1*b + (-1)*Ap and (TREE) r.r              numerics.F90:97-99
+// Prepare: r = 1*b + (-1)*Ap and (TREE) the workgroup's r.r      numerics.F90:97-98
 template <bool TREE>
 __global__ __launch_bounds__(kVecThreads) void k_cg_prep_r(const double *__restrict__ b,
                                                            const double *__restrict__ Ap,
-                                                           double *__restrict__ r, int n,
-                                                           CgState *st, double *rr_hist,
-                                                           double *partials)
+                                                           double *__restrict__ r, int n, double *partials)
 {
     __shared__ double red[kVecThreads / kWave];
     int64_t i0, i1;
@@ -128,24 +126,17 @@ __global__ __launch_bounds__(kVecThreads) void k_cg_prep_r(const double *__restr
         r[i] = ri;
         dot = fma(ri, ri, dot);
     }
-    if (TREE) {
-        dot = block_sum<kVecThreads>(dot, red);
-        double tot;
-        if (grid_sum<kVecThreads>(dot, partials, &st->counter[1], gridDim.x, lb, red, &tot) && threadIdx.x == 0)
-            cg_after_rr(st, rr_hist, 0, tot);
-    }
+    if (TREE) store_partial(block_sum<kVecThreads>(dot, red), partials);
 }
 
-// r = r + (-alpha)*Ap and (TREE) r.r -> beta, test, k++    numerics.F90:139, :117-119
+// r = r + (-alpha)*Ap and (TREE) the workgroup's r.r      numerics.F90:139, :117
 template <bool TREE>
 __global__ __launch_bounds__(kVecThreads) void k_cg_update_r(double *__restrict__ r,
                                                              const double *__restrict__ Ap, int n,
-                                                             CgState *st, double *rr_hist,
-                                                             double *partials)
+                                                             const CgState *st, double *partials)
 {
     __shared__ double red[kVecThreads / kWave];
-    const int k = st->k;
-    if (k >= st->stop_at) return;
+    if (st->k >= st->stop_at) return;
     const double nalpha = -st->alpha;
     int64_t i0, i1;
     int lb;
@@ -157,12 +148,19 @@ __global__ __launch_bounds__(kVecThreads) void k_cg_update_r(double *__restrict_
         r[i] = ri;
         dot = fma(ri, ri, dot);
     }
-    if (TREE) {
-        dot = block_sum<kVecThreads>(dot, red);
-        double tot;
-        if (grid_sum<kVecThreads>(dot, partials, &st->counter[2], gridDim.x, lb, red, &tot) && threadIdx.x == 0)
-            cg_after_rr(st, rr_hist, k, tot);
-    }
+    if (TREE) store_partial(block_sum<kVecThreads>(dot, red), partials);
+}
+
+// r.r from the partials -> beta, test, k++ (PREP: rtrans of Prepare)   numerics.F90:98-99, :117-119
+template <bool PREP>
+__global__ __launch_bounds__(kFinishThreads) void k_cg_rr_finish(const double *__restrict__ partials, int nparts,
+                                                                 CgState *st, double *rr_hist)
+{
+    __shared__ double red[kFinishThreads / kWave];
+    const int k = PREP ? 0 : st->k;
+    if (!PREP && k >= st->stop_at) return;
+    const double rr = sum_partials(partials, nparts, red);
+    if (threadIdx.x == 0) cg_after_rr(st, rr_hist, k, rr);
 }
 
 // Top of iteration k: k == 1: p = r (numerics.F90:114-115); else x = x + alpha_{k-1} p_{k-1}
@@ -279,9 +277,7 @@ __global__ __launch_bounds__(kWave) void k_cg_serial_rr(const double *r, int n, 
 
 // ---- standalone tree dot and vector ops -----------------------------------------------------
 __global__ __launch_bounds__(kVecThreads) void k_dot_tree(const double *__restrict__ x,
-                                                          const double *__restrict__ y, int n,
-                                                          double *partials, unsigned *counter,
-                                                          double *out)
+                                                          const double *__restrict__ y, int n, double *partials)
 {
     __shared__ double red[kVecThreads / kWave];
     int64_t i0, i1;
@@ -289,10 +285,14 @@ __global__ __launch_bounds__(kVecThreads) void k_dot_tree(const double *__restri
     elem_range(n, i0, i1, lb);
     double dot = 0.0;
     for (int64_t i = i0 + threadIdx.x; i < i1; i += kVecThreads) dot = fma(x[i], y[i], dot);
-    dot = block_sum<kVecThreads>(dot, red);
-    double tot;
-    if (grid_sum<kVecThreads>(dot, partials, counter, gridDim.x, lb, red, &tot) && threadIdx.x == 0)
-        *out = tot;
+    store_partial(block_sum<kVecThreads>(dot, red), partials);
+}
+__global__ __launch_bounds__(kFinishThreads) void k_dot_finish(const double *__restrict__ partials, int nparts,
+                                                               double *out)
+{
+    __shared__ double red[kFinishThreads / kWave];
+    const double s = sum_partials(partials, nparts, red);
+    if (threadIdx.x == 0) *out = s;
 }
 
 // max_i |v1_i - v2_i| (numerics.F90:178-183): per-thread maxima, a wavefront/workgroup max, and
@@ -475,8 +475,9 @@ static int launch_dot(hpccg_ctx *c, int n, const double *x, const double *y, dou
     if (c->dot_mode == HPCCG_DOT_SEQUENTIAL) {
         hipLaunchKernelGGL(k_dot_serial, dim3(1), dim3(kWave), 0, c->stream, x, y, n, out_dev);
     } else {
-        hipLaunchKernelGGL(k_dot_tree, dim3(vec_grid(c, n)), dim3(kVecThreads), 0, c->stream, x, y, n, c->partials,
-                           &c->state->counter[3], out_dev);
+        const int G = vec_grid(c, n);
+        hipLaunchKernelGGL(k_dot_tree, dim3(G), dim3(kVecThreads), 0, c->stream, x, y, n, c->partials);
+        hipLaunchKernelGGL(k_dot_finish, dim3(1), dim3(kFinishThreads), 0, c->stream, c->partials, G, out_dev);
     }
     HPCCG_HIP_TRY(hipGetLastError());
     return 0;
@@ -509,12 +510,14 @@ static int enqueue_iterations(hpccg_ctx *c, double *x, int count)
         launch_cg_spmv(c, tree, c->p, c->Ap);
         if (tree) {
             hipLaunchKernelGGL(k_cg_update_r<true>, dim3(Gv), dim3(kVecThreads), 0, c->stream, c->r, c->Ap, n,
-                               c->state, c->rr_hist, c->partials);
+                               c->state, c->partials);
+            hipLaunchKernelGGL(k_cg_rr_finish<false>, dim3(1), dim3(kFinishThreads), 0, c->stream, c->partials, Gv,
+                               c->state, c->rr_hist);
         } else {
             hipLaunchKernelGGL(k_cg_serial_pap, dim3(1), dim3(kWave), 0, c->stream, c->p, c->Ap, n, c->state,
                                c->rr_hist);
             hipLaunchKernelGGL(k_cg_update_r<false>, dim3(Gv), dim3(kVecThreads), 0, c->stream, c->r, c->Ap, n,
-                               c->state, c->rr_hist, c->partials);
+                               c->state, c->partials);
             hipLaunchKernelGGL(k_cg_serial_rr, dim3(1), dim3(kWave), 0, c->stream, c->r, n, c->state, c->rr_hist, 0);
         }
     }
@@ -575,10 +578,12 @@ static int cg_device(hpccg_ctx *c, const double *b, double *x, int max_iter, dou
     HPCCG_TRY(launch_spmv(c, x, c->Ap));
     if (tree) {
         hipLaunchKernelGGL(k_cg_prep_r<true>, dim3(vec_grid(c, n)), dim3(kVecThreads), 0, c->stream, b, c->Ap, c->r,
-                           n, c->state, c->rr_hist, c->partials);
+                           n, c->partials);
+        hipLaunchKernelGGL(k_cg_rr_finish<true>, dim3(1), dim3(kFinishThreads), 0, c->stream, c->partials,
+                           vec_grid(c, n), c->state, c->rr_hist);
     } else {
         hipLaunchKernelGGL(k_cg_prep_r<false>, dim3(vec_grid(c, n)), dim3(kVecThreads), 0, c->stream, b, c->Ap,
-                           c->r, n, c->state, c->rr_hist, c->partials);
+                           c->r, n, c->partials);
         hipLaunchKernelGGL(k_cg_serial_rr, dim3(1), dim3(kWave), 0, c->stream, c->r, n, c->state, c->rr_hist, 1);
     }
     HPCCG_HIP_TRY(hipGetLastError());
@@ -645,8 +650,9 @@ int cg_trace(hpccg_ctx *c, const double *b, double *x, int max_iter, double tol,
     *c->state_h = init;
     HPCCG_HIP_TRY(hipMemcpyAsync(c->state, c->state_h, sizeof(CgState), hipMemcpyHostToDevice, c->stream));
     HPCCG_TRY(launch_spmv(c, x, c->Ap));
-    hipLaunchKernelGGL(k_cg_prep_r<true>, dim3(Gv), dim3(kVecThreads), 0, c->stream, b, c->Ap, c->r, n, c->state,
-                       c->rr_hist, c->partials);
+    hipLaunchKernelGGL(k_cg_prep_r<true>, dim3(Gv), dim3(kVecThreads), 0, c->stream, b, c->Ap, c->r, n, c->partials);
+    hipLaunchKernelGGL(k_cg_rr_finish<true>, dim3(1), dim3(kFinishThreads), 0, c->stream, c->partials, Gv, c->state,
+                       c->rr_hist);
     HPCCG_HIP_TRY(hipGetLastError());
     // p_0 = r_0 and alpha = 0 make the hook's x + alpha p equal x_0
     HPCCG_HIP_TRY(hipMemcpyAsync(c->p, c->r, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
@@ -660,7 +666,9 @@ int cg_trace(hpccg_ctx *c, const double *b, double *x, int max_iter, double tol,
         hipLaunchKernelGGL(k_cg_update_p, dim3(Gv), dim3(kVecThreads), 0, c->stream, c->r, c->p, x, n, c->state);
         launch_cg_spmv(c, true, c->p, c->Ap);
         hipLaunchKernelGGL(k_cg_update_r<true>, dim3(Gv), dim3(kVecThreads), 0, c->stream, c->r, c->Ap, n, c->state,
-                           c->rr_hist, c->partials);
+                           c->partials);
+        hipLaunchKernelGGL(k_cg_rr_finish<false>, dim3(1), dim3(kFinishThreads), 0, c->stream, c->partials, Gv,
+                           c->state, c->rr_hist);
         HPCCG_HIP_TRY(hipGetLastError());
         HPCCG_HIP_TRY(hipStreamSynchronize(c->stream));
         if (hook(k)) break;

@@ -527,15 +527,14 @@ __global__ __launch_bounds__(kFinishThreads) void kf_spmv_finish(const double *_
     }
 }
 
-// x += alpha p (FP64), r -= alpha Ap (FP32), r.r; requests a true residual when due
+// x += alpha p (FP64), r -= alpha Ap (FP32), and the workgroup's r.r
 __global__ __launch_bounds__(kVecThreads) void kf_update_xr(double *__restrict__ x, float *__restrict__ r,
                                                             const float *__restrict__ p,
-                                                            const float *__restrict__ Ap, int n, FcgState *st,
-                                                            double *hist_rec, double *partials)
+                                                            const float *__restrict__ Ap, int n, const FcgState *st,
+                                                            double *partials)
 {
     __shared__ double red[kVecThreads / kWave];
-    const int k = st->k;
-    if (k >= st->stop_at) return;
+    if (st->k >= st->stop_at) return;
     const double alpha = st->alpha;
     int64_t i0, i1;
     int lb;
@@ -547,9 +546,18 @@ __global__ __launch_bounds__(kVecThreads) void kf_update_xr(double *__restrict__
         r[i] = ri;
         dot = fma((double)ri, (double)ri, dot);
     }
-    dot = block_sum<kVecThreads>(dot, red);
-    double rr;
-    if (grid_sum<kVecThreads>(dot, partials, &st->counter[2], gridDim.x, lb, red, &rr) && threadIdx.x == 0) {
+    store_partial(block_sum<kVecThreads>(dot, red), partials);
+}
+
+// r.r: network scales; requests a true residual when due
+__global__ __launch_bounds__(kFinishThreads) void kf_xr_finish(const double *__restrict__ partials, int nparts, int n,
+                                                               FcgState *st, double *hist_rec)
+{
+    __shared__ double red[kFinishThreads / kWave];
+    const int k = st->k;
+    if (k >= st->stop_at) return;
+    const double rr = sum_partials(partials, nparts, red);
+    if (threadIdx.x == 0) {
         hist_rec[k] = rr;
         st->rr = rr;
         const double rms = sqrt(rr / (double)n);
@@ -563,11 +571,10 @@ __global__ __launch_bounds__(kVecThreads) void kf_update_xr(double *__restrict__
     }
 }
 
-// z.Ap and z.r -> beta = -(z.Ap)/(p.Ap) (flexible CG, Notay 2000), rz = z.r; then p = z + beta p
-// is applied by kf_update_p.  PREP: rz = z.r only (first direction p = z).
+// z.Ap and z.r of the workgroup (two partials per workgroup)
 template <bool PREP>
 __global__ __launch_bounds__(kVecThreads) void kf_dots(const float *__restrict__ z, const float *__restrict__ Ap,
-                                                       const float *__restrict__ r, int n, FcgState *st,
+                                                       const float *__restrict__ r, int n, const FcgState *st,
                                                        double *partials)
 {
     __shared__ double red[kVecThreads / kWave];
@@ -584,25 +591,29 @@ __global__ __launch_bounds__(kVecThreads) void kf_dots(const float *__restrict__
     }
     za = block_sum<kVecThreads>(za, red);
     zr = block_sum<kVecThreads>(zr, red);
-    __shared__ int is_last;
     if (threadIdx.x == 0) {
-        partials[2 * lb] = za;
-        partials[2 * lb + 1] = zr;
-        __threadfence();
-        is_last = atomicAdd(&st->counter[3], 1u) == gridDim.x - 1u;
+        partials[2 * blockIdx.x] = za;
+        partials[2 * blockIdx.x + 1] = zr;
     }
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();
+}
+
+// beta = -(z.Ap)/(p.Ap) (flexible CG, Notay 2000), rz = z.r; p = z + beta p follows in
+// kf_update_p.  PREP: rz = z.r only (first direction p = z).
+template <bool PREP>
+__global__ __launch_bounds__(kFinishThreads) void kf_dots_finish(const double *__restrict__ partials, int nparts,
+                                                                 FcgState *st)
+{
+    __shared__ double red[kFinishThreads / kWave];
+    if (!PREP && st->k >= st->stop_at) return;
+    if (PREP && st->stop_at <= 1) return;
     double a = 0.0, b = 0.0;
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += kVecThreads) {
-        a += load_agent(partials + 2 * i);
-        b += load_agent(partials + 2 * i + 1);
+    for (int i = threadIdx.x; i < nparts; i += kFinishThreads) {
+        a += partials[2 * i];
+        b += partials[2 * i + 1];
     }
-    a = block_sum<kVecThreads>(a, red);
-    b = block_sum<kVecThreads>(b, red);
+    a = block_sum<kFinishThreads>(a, red);
+    b = block_sum<kFinishThreads>(b, red);
     if (threadIdx.x == 0) {
-        __hip_atomic_store(&st->counter[3], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         st->beta = PREP ? 0.0 : -a / st->pap;
         st->rz = b;
         if (!(b > 0.0) || !isfinite(b) || !isfinite(st->beta)) {  // z not a descent direction
@@ -1075,7 +1086,9 @@ int hpccg_fcg_learned_dev(hpccg_ctx *c, int32_t nx, int32_t ny, int32_t nz, cons
                            A.red_part);
         hipLaunchKernelGGL(kf_spmv_finish, dim3(1), dim3(kFinishThreads), 0, c->stream, A.red_part, Gs, F->st);
         hipLaunchKernelGGL(kf_update_xr, dim3(Gv), dim3(kVecThreads), 0, c->stream, x, F->r, F->p, F->Ap, n, F->st,
-                           hrec, c->partials);
+                           c->partials);
+        hipLaunchKernelGGL(kf_xr_finish, dim3(1), dim3(kFinishThreads), 0, c->stream, c->partials, Gv, n, F->st,
+                           hrec);
         hipLaunchKernelGGL(kf_true<false>, dim3(Gs), dim3(kSellThreads), 0, c->stream, sell16(A), x, b, F->r, F->st,
                            A.red_part);
         hipLaunchKernelGGL(kf_true_finish<false>, dim3(1), dim3(kFinishThreads), 0, c->stream, A.red_part, Gs, n,
@@ -1083,6 +1096,7 @@ int hpccg_fcg_learned_dev(hpccg_ctx *c, int32_t nx, int32_t ny, int32_t nz, cons
         forward_fused(c, F->r, s_in, F->z, s_out, g, skip);
         hipLaunchKernelGGL(kf_dots<false>, dim3(Gv), dim3(kVecThreads), 0, c->stream, F->z, F->Ap, F->r, n, F->st,
                            c->partials);
+        hipLaunchKernelGGL(kf_dots_finish<false>, dim3(1), dim3(kFinishThreads), 0, c->stream, c->partials, Gv, F->st);
         hipLaunchKernelGGL(kf_update_p<false>, dim3(Gv), dim3(kVecThreads), 0, c->stream, F->z, F->p, n, F->st);
     };
 
@@ -1095,6 +1109,7 @@ int hpccg_fcg_learned_dev(hpccg_ctx *c, int32_t nx, int32_t ny, int32_t nz, cons
     forward_fused(c, F->r, s_in, F->z, s_out, g, skip);
     hipLaunchKernelGGL(kf_dots<true>, dim3(Gv), dim3(kVecThreads), 0, c->stream, F->z, F->Ap, F->r, n, F->st,
                        c->partials);
+    hipLaunchKernelGGL(kf_dots_finish<true>, dim3(1), dim3(kFinishThreads), 0, c->stream, c->partials, Gv, F->st);
     hipLaunchKernelGGL(kf_update_p<true>, dim3(Gv), dim3(kVecThreads), 0, c->stream, F->z, F->p, n, F->st);
     HPCCG_HIP_TRY(hipGetLastError());
 

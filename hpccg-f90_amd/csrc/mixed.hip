@@ -150,17 +150,14 @@ __global__ __launch_bounds__(kFinishThreads) void km_spmv_finish(const double *_
     }
 }
 
-// x = x + alpha p (FP64), r = r - alpha Ap (FP32 store), recursive r.r; decides whether the
-// true residual is formed next (every replace_every iterations, when the recursive norm passes
-// the tolerance, and at max_iter).
+// x = x + alpha p (FP64), r = r - alpha Ap (FP32 store), and the workgroup's recursive r.r
 __global__ __launch_bounds__(kVecThreads) void km_update_xr(double *__restrict__ x, float *__restrict__ r,
                                                             const float *__restrict__ p,
-                                                            const float *__restrict__ Ap, int n, MixState *st,
-                                                            double *hist_rec, double *partials)
+                                                            const float *__restrict__ Ap, int n, const MixState *st,
+                                                            double *partials)
 {
     __shared__ double red[kVecThreads / kWave];
-    const int k = st->k;
-    if (k >= st->stop_at) return;
+    if (st->k >= st->stop_at) return;
     const double alpha = st->alpha;
     int64_t i0, i1;
     int lb;
@@ -172,9 +169,19 @@ __global__ __launch_bounds__(kVecThreads) void km_update_xr(double *__restrict__
         r[i] = ri;
         dot = fma((double)ri, (double)ri, dot);
     }
-    dot = block_sum<kVecThreads>(dot, red);
-    double rr;
-    if (grid_sum<kVecThreads>(dot, partials, &st->counter[2], gridDim.x, lb, red, &rr) && threadIdx.x == 0) {
+    store_partial(block_sum<kVecThreads>(dot, red), partials);
+}
+
+// recursive r.r; decides whether the true residual is formed next (every replace_every
+// iterations, when the recursive norm passes the tolerance, and at max_iter)
+__global__ __launch_bounds__(kFinishThreads) void km_xr_finish(const double *__restrict__ partials, int nparts,
+                                                               MixState *st, double *hist_rec)
+{
+    __shared__ double red[kFinishThreads / kWave];
+    const int k = st->k;
+    if (k >= st->stop_at) return;
+    const double rr = sum_partials(partials, nparts, red);
+    if (threadIdx.x == 0) {
         hist_rec[k] = rr;
         const bool check = (st->replace_every > 0 && k % st->replace_every == 0) || sqrt(rr) < st->tol ||
                            k >= st->max_iter;
@@ -314,8 +321,9 @@ static int mixed_enqueue(hpccg_ctx *c, const double *b, double *x, int count)
     for (int it = 0; it < count; ++it) {
         hipLaunchKernelGGL(km_update_p, dim3(Gv), dim3(kVecThreads), 0, c->stream, m.r, m.p, n, st);
         launch_km_spmv(c, m.p, m.Ap);
-        hipLaunchKernelGGL(km_update_xr, dim3(Gv), dim3(kVecThreads), 0, c->stream, x, m.r, m.p, m.Ap, n, st, hrec,
+        hipLaunchKernelGGL(km_update_xr, dim3(Gv), dim3(kVecThreads), 0, c->stream, x, m.r, m.p, m.Ap, n, st,
                            c->partials);
+        hipLaunchKernelGGL(km_xr_finish, dim3(1), dim3(kFinishThreads), 0, c->stream, c->partials, Gv, st, hrec);
         launch_km_true<false>(c, b, x);
     }
     HPCCG_HIP_TRY(hipGetLastError());
