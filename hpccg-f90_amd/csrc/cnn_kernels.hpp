@@ -341,19 +341,24 @@ __global__ __launch_bounds__(256) void k_wgrad(const uint16_t *__restrict__ dout
 
 // ---- fused inference: the three layers in one kernel -------------------------------------------
 // A workgroup owns a 16 x 16 (x, y) column of output voxels and a range of z-planes and marches
-// along z.  Step i stages input plane i (rounded to BF16) into a three-plane window and then
-//   layer 1: computes h1 plane i-1 from the window (one MFMA per 16 voxels, K = 27 taps),
-//   layer 2: reads h1 plane i-1 ONCE -- five operands per 16 voxels, one per in-plane tap pair
-//            -- and adds it to the three h2 planes it touches (15 MFMAs), h2 plane i-2 being
-//            complete afterwards;
-//   layer 3: reads h2 plane i-2 once the same way (five MFMAs, the three plane-offset partials
-//            in rows 0..2) and completes output plane i-3.
-// The h2 and output partial sums of unfinished planes stay in registers, rotating through three
-// slots (the loop body is unrolled three times so that the slots are compile-time registers),
-// so only one plane of h1 and one of h2 are kept in LDS, and every LDS operand feeds three
-// MFMAs.  h1 is computed on a 20 x 20 and h2 on an 18 x 18 footprint (the halos the next
-// layer needs).  Each accumulator receives its K-steps in the order of the layer-wise k_conv
-// (z-major), so the output is bitwise the three-kernel forward pass's.
+// along z, one plane per step.  In step i every wave does its share of
+//   layer 1: h1 plane i-1 from the input window (planes i-2 .. i; one MFMA per 16 voxels,
+//            K = 27 taps);
+//   layer 2: reads h1 plane i-2 ONCE -- five operands per 16 voxels, one per in-plane tap
+//            pair -- and adds it to the three h2 planes it touches (15 MFMAs), h2 plane i-3
+//            being complete afterwards;
+//   layer 3: reads h2 plane i-4 once the same way (five MFMAs, the three plane-offset partials
+//            in rows 0..2) and completes output plane i-5;
+//   input:   stores plane i+1 (rounded to BF16, loaded three steps earlier) into a four-plane
+//            ring and requests plane i+4.
+// The layers of one step touch different planes -- h1 and h2 are double-buffered in LDS -- so a
+// step needs a single barrier and a wave's three layers overlap.  The h2 and output partial
+// sums of unfinished planes stay in registers, rotating through three slots (the loop body is
+// unrolled three times so that the slots are compile-time registers).  h1 is computed on a
+// 20 x 20 and h2 on an 18 x 18 footprint (the halos the next layer needs).  Every LDS address
+// and grid-boundary test that does not depend on the plane is formed once, before the march.
+// Each accumulator receives its K-steps in the order of the layer-wise k_conv (z-major), so the
+// output is bitwise the three-kernel forward pass's.
 constexpr int FX = 16, FY = 16;             // output footprint per workgroup
 constexpr int F2X = FX + 2, F2Y = FY + 2;   // h2 footprint (324 voxels, 21 groups of 16)
 constexpr int F1X = FX + 4, F1Y = FY + 4;   // h1 footprint (400 voxels, 25 groups)
@@ -361,15 +366,16 @@ constexpr int F0X = FX + 6, F0Y = FY + 6;   // input footprint (484 voxels)
 constexpr int F2V = F2X * F2Y, F1V = F1X * F1Y, F0V = F0X * F0Y;
 constexpr int F2G = (F2V + 15) / 16, F1G = F1V / 16, F3G = FX * FY / 16;
 constexpr int FWAVES = 8, FTHREADS = 64 * FWAVES;  // waves per fused workgroup
-constexpr int G2W = (F2G + FWAVES - 1) / FWAVES, G3W = F3G / FWAVES;  // layer-2 / -3 voxel groups per wave
+constexpr int G1W = (F1G + FWAVES - 1) / FWAVES;   // voxel groups per wave: layer 1 (4),
+constexpr int G2W = (F2G + FWAVES - 1) / FWAVES;   //   layer 2 (3),
+constexpr int G3W = F3G / FWAVES;                  //   layer 3 (2)
+constexpr int TPL = F0Y * F1X;                     // x-triplets per input-window plane (440)
+constexpr int VS = 16;  // h1 / h2 voxel record in LDS: 16 BF16 channels (32 bytes)
+constexpr int H1S = F1V * VS + 8, H2S = F2V * VS + 8;  // plane buffers + 8 zero channels
 
 template <int K>
 using Slot = std::integral_constant<int, K>;
 
-// Layer 1 reads its taps as three x-triplets per lane (the input window keeps its planes in
-// order, so the offsets are fixed per lane).  Every LDS address and grid-boundary test that
-// does not depend on the plane is formed once, before the march: a lane's voxel groups are
-// the same on every plane.
 __global__ __launch_bounds__(FTHREADS) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_fused_fwd(
     const float *__restrict__ in, const float *__restrict__ in_scale, const bf16x8 *__restrict__ pk1,
     const bf16x8 *__restrict__ pk2, const bf16x8 *__restrict__ pk3f, float *__restrict__ out,
@@ -377,22 +383,24 @@ __global__ __launch_bounds__(FTHREADS) __attribute__((amdgpu_waves_per_eu(2, 2))
     const int *__restrict__ skip)
 {
     if (skip && *skip) return;
-    // input window, planes i-2, i-1, i, stored as x-triplets: T[plane][row][x] holds the input at
-    // window columns x, x+1, x+2 (one 8-byte read per tap line of layer 1)
-    __shared__ __attribute__((aligned(16))) u32x2 T[3][F0Y][F1X];
-    // one plane of h1 / h2, each followed by 8 zero channels (the operand of the empty tenth
-    // tap slot)
-    __shared__ __attribute__((aligned(16))) uint16_t H1[F1V * 16 + 8];
-    __shared__ __attribute__((aligned(16))) uint16_t H2[F2V * 16 + 8];
+    // input ring: plane p in slot p & 3, as x-triplets: T[slot][row * F1X + x] holds the input
+    // at window row `row`, columns x, x+1, x+2 (one 8-byte read per tap line of layer 1)
+    __shared__ __attribute__((aligned(16))) u32x2 T[4][TPL];
+    // h1 / h2 plane p in buffer p & 1, each followed by 8 zero channels (the operand of the
+    // empty tenth tap slot)
+    __shared__ __attribute__((aligned(16))) uint16_t H1[2][H1S];
+    __shared__ __attribute__((aligned(16))) uint16_t H2[2][H2S];
     const int txy = blockIdx.x % (tiles_x * tiles_y), zc = blockIdx.x / (tiles_x * tiles_y);
     const int ox = (txy % tiles_x) * FX, oy = (txy / tiles_x) * FY;
     const int z0 = zc * zchunk, z1 = min(z0 + zchunk, nz);
     const float scale = in_scale ? in_scale[0] : 1.0f;
     const float oscale = out_scale ? out_scale[0] : 1.0f;
-    const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+    const int lane = threadIdx.x % 64, w = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
     const int c = lane / 16, n = lane % 16;
     const bool hi = c >= 2;        // operand rows of the second tap of each pair
     const int half = (c & 1) * 8;  // channel half
+    const int hiv = hi ? VS : 0;
+    const int64_t plane = (int64_t)nx * ny;
     const bf16x8 A1 = pk1[lane];
     bf16x8 A2[NSTEP16], A3[NSTEP3F];
 #pragma unroll
@@ -400,77 +408,77 @@ __global__ __launch_bounds__(FTHREADS) __attribute__((amdgpu_waves_per_eu(2, 2))
 #pragma unroll
     for (int s = 0; s < NSTEP3F; ++s) A3[s] = pk3f[s * 64 + lane];
     if (threadIdx.x < 8) {
-        H1[F1V * 16 + threadIdx.x] = 0;
-        H2[F2V * 16 + threadIdx.x] = 0;
+        H1[0][F1V * VS + threadIdx.x] = 0;
+        H1[1][F1V * VS + threadIdx.x] = 0;
+        H2[0][F2V * VS + threadIdx.x] = 0;
+        H2[1][F2V * VS + threadIdx.x] = 0;
     }
-    const int64_t plane = (int64_t)nx * ny;
 
     // ---- plane-independent per-lane facts ----
-    // layer 1: group k is g = w + 8k, voxel v = 16 g + n of the 20 x 20 footprint (T row = v)
-    constexpr int G1W = (F1G + FWAVES - 1) / FWAVES;
+    // layer 1: group k is g = w + 8k, voxel v = 16 g + n of the 20 x 20 footprint (T row = v);
+    // tap lines 2c, 2c+1 and 8 of this lane's operand row, window planes lX / 3
     const int v1 = 16 * w + n;
-    const int lA = 2 * c, lB = 2 * c + 1;  // layer-1 tap lines of this lane's operand row
-    const u32x2 *TA = &T[0][0][0] + ((lA / 3) * F0Y + lA % 3) * F1X + v1;
-    const u32x2 *TB = &T[0][0][0] + ((lB / 3) * F0Y + lB % 3) * F1X + v1;
-    const u32x2 *TC = &T[0][0][0] + (2 * F0Y + 2) * F1X + v1;
-    uint16_t *H1w = H1 + v1 * 16 + 4 * c;
-    unsigned xy1 = 0;  // bit k: group k exists and its voxel lies inside the grid (x, y)
+    const int lA = 2 * c, lB = 2 * c + 1;
+    const int rA = (lA % 3) * F1X + v1, rB = (lB % 3) * F1X + v1, rC = 2 * F1X + v1;
+    const int pA = lA / 3, pB = lB / 3;
+    unsigned ok1 = 0;  // bit k: group k exists and its voxel lies inside the grid (x, y)
 #pragma unroll
     for (int k = 0; k < G1W; ++k) {
         const int g = w + FWAVES * k, v = v1 + 16 * FWAVES * k;
         const int gx = ox - 2 + v % F1X, gy = oy - 2 + v / F1X;
-        if (g < F1G && gx >= 0 && gx < nx && gy >= 0 && gy < ny) xy1 |= 1u << k;
+        if (g < F1G && gx >= 0 && gx < nx && gy >= 0 && gy < ny) ok1 |= 1u << k;
     }
-    // layers 2 and 3 read operand pair p at base + off(2p) (+ the second tap for hi lanes):
-    // off(t9) = (t9 / 3) RX + t9 % 3 voxels; the hi lanes add 1 voxel (pairs 0, 2, 3), RX - 2
-    // (pair 1), or read the zero channels (pair 4)
-    const int hi16 = hi ? 16 : 0;
-    int b2[G2W];       // layer-2 operand base of group k (hi offset included), in uint16 units
-    unsigned xy2 = 0;  // bit k: h2 voxel of group k exists and lies inside the grid (x, y)
+    // layer 2: group k is g = w + 8k; operand base (hi offset included) and validity
+    int b2[G2W];
+    unsigned ok2 = 0;
 #pragma unroll
     for (int k = 0; k < G2W; ++k) {
         const int g = w + FWAVES * k, v = 16 * g + n, vc = min(v, F2V - 1), hx = vc % F2X, hy = vc / F2X;
-        b2[k] = (hy * F1X + hx) * 16 + half + hi16;
+        b2[k] = (hy * F1X + hx) * VS + half + hiv;
         const int gx = ox - 1 + hx, gy = oy - 1 + hy;
-        if (g < F2G && v < F2V && gx >= 0 && gx < nx && gy >= 0 && gy < ny) xy2 |= 1u << k;
+        if (g < F2G && v < F2V && gx >= 0 && gx < nx && gy >= 0 && gy < ny) ok2 |= 1u << k;
     }
-    uint16_t *H2w = H2 + v1 * 16 + 4 * c;
-    const int b3 = ((w * F2X) + n) * 16 + half + hi16;  // layer-3 operand base of group 0
-    const int64_t ob = (int64_t)(oy + w) * nx + ox + n;  // output offset of group 0 in a plane
-    unsigned xy3 = 0;
+    // layer 3: group k is output row y = w + 8k
+    const int b3 = ((w * F2X) + n) * VS + half + hiv;
+    const int64_t ob = (int64_t)(oy + w) * nx + ox + n;
+    unsigned ok3 = 0;
 #pragma unroll
     for (int k = 0; k < G3W; ++k)
-        if (c == 0 && ox + n < nx && oy + w + FWAVES * k < ny) xy3 |= 1u << k;
+        if (c == 0 && ox + n < nx && oy + w + FWAVES * k < ny) ok3 |= 1u << k;
 
+    // input loader: thread t < TPL owns triplet t (window row t / F1X, columns t % F1X .. +2).
+    // The three loads read clamped, always valid addresses without branches, so they stay in
+    // flight until the plane is stored three steps later; the zero padding outside the grid and
+    // the BF16 rounding are applied then.
+    const bool trip = threadIdx.x < TPL;
+    const int tgy = oy - 3 + (int)threadIdx.x / F1X, tgx0 = ox - 3 + (int)threadIdx.x % F1X;
+    const bool trow = trip && tgy >= 0 && tgy < ny;
+    const bool tx0 = tgx0 >= 0 && tgx0 < nx, tx1 = tgx0 + 1 >= 0 && tgx0 + 1 < nx, tx2 = tgx0 + 2 >= 0 && tgx0 + 2 < nx;
+    const float *trowp = in + (int64_t)min(max(tgy, 0), ny - 1) * nx;
+    const int cx0 = min(max(tgx0, 0), nx - 1), cx1 = min(max(tgx0 + 1, 0), nx - 1), cx2 = min(max(tgx0 + 2, 0), nx - 1);
+    auto fetch = [&](int ip) -> f32x4 {
+        const float *row = trowp + (int64_t)min(max(ip, 0), nz - 1) * plane;
+        return f32x4{row[cx0], row[cx1], row[cx2], 0.f};
+    };
+    auto to_triplet = [&](int ip, f32x4 f) -> u32x2 {
+        const bool ok = trow && ip >= 0 && ip < nz;
+        u32x2 t3;
+        t3[0] = pack_bf16(ok && tx0 ? f[0] * scale : 0.f, ok && tx1 ? f[1] * scale : 0.f);
+        t3[1] = pack_bf16(ok && tx2 ? f[2] * scale : 0.f, 0.f);
+        return t3;
+    };
     // five B operands of a voxel group whose operand base (hi offset included) is `base`, in a
-    // one-plane buffer H of row length RX followed by zero channels at ZOFF
+    // one-plane buffer H of row length RX whose zero channels start at ZOFF: off(t9) =
+    // (t9 / 3) RX + t9 % 3 voxels; hi lanes add 1 voxel (pairs 0, 2, 3) or RX - 2 (pair 1), or
+    // read the zero channels (pair 4)
     auto load_pairs = [&](const uint16_t *H, auto RXc, int ZOFF, int base, u32x4 *raw) {
         constexpr int RX = decltype(RXc)::value;
         const uint16_t *pb = H + base;
         raw[0] = *(const u32x4 *)(pb);
-        raw[1] = *(const u32x4 *)(pb + (hi ? (RX - 3) * 16 : 0) + 2 * 16);
-        raw[2] = *(const u32x4 *)(pb + (RX + 1) * 16);
-        raw[3] = *(const u32x4 *)(pb + (2 * RX) * 16);
-        raw[4] = *(const u32x4 *)(hi ? H + ZOFF : pb + (2 * RX + 2) * 16);
-    };
-    // triplet e of a plane: window row e / F1X, columns e % F1X .. +2 (440 per plane, one per thread)
-    const bool trip = threadIdx.x < F0Y * F1X;
-    const int tgy = oy - 3 + (int)threadIdx.x / F1X, tgx0 = ox - 3 + (int)threadIdx.x % F1X;
-    const bool trow = trip && tgy >= 0 && tgy < ny;
-    auto load_triplet = [&](int ip) -> u32x2 {
-        u32x2 t3;
-        t3[0] = t3[1] = 0u;
-        if (!trow || ip < 0 || ip >= nz) return t3;
-        const float *row = in + ip * plane + (int64_t)tgy * nx;
-        float u[3];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const int gx = tgx0 + k;
-            u[k] = (gx >= 0 && gx < nx) ? row[gx] * scale : 0.f;
-        }
-        t3[0] = pack_bf16(u[0], u[1]);
-        t3[1] = pack_bf16(u[2], 0.f);
-        return t3;
+        raw[1] = *(const u32x4 *)(pb + (hi ? (RX - 3) * VS : 0) + 2 * VS);
+        raw[2] = *(const u32x4 *)(pb + (RX + 1) * VS);
+        raw[3] = *(const u32x4 *)(pb + (2 * RX) * VS);
+        raw[4] = *(const u32x4 *)(hi ? H + ZOFF : pb + (2 * RX + 2) * VS);
     };
 
     f32x4 acc2[3][G2W];
@@ -482,53 +490,31 @@ __global__ __launch_bounds__(FTHREADS) __attribute__((amdgpu_waves_per_eu(2, 2))
 #pragma unroll
         for (int k = 0; k < G3W; ++k) acc3[r][k] = 0.f;
     }
-    u32x2 pre = load_triplet(z0 - 3);
+    // input planes are requested three steps before they are stored (step i stores plane i+1,
+    // loaded during step i-3), so the global latency hides behind three steps of work
+    f32x4 pre0 = fetch(z0 - 3), pre1 = fetch(z0 - 2), pre2 = fetch(z0 - 1);
 
-    // one z-step; slot R holds the partial sums of the planes the step starts (h2 plane i,
-    // output plane i-1); slot R+2 those it continues; slot R+1 those it completes
+    // step i; rotating slot R holds the partial sums the step starts (h2 plane i-1, output plane
+    // i-3), R+2 those it continues, R+1 those it completes
     auto step = [&](auto R, int i) {
         constexpr int NXT = decltype(R)::value, CUR = (NXT + 2) % 3, PRV = (NXT + 1) % 3;
-        // shift the window (each thread owns the same triplet of all three planes) and store
-        // plane i, prefetched during the previous step; then prefetch plane i + 1
-        if (trip) {
-            u32x2 *Tv = &T[0][0][0] + threadIdx.x;
-            const u32x2 a1 = Tv[F0Y * F1X], a2 = Tv[2 * F0Y * F1X];
-            Tv[0] = a1;
-            Tv[F0Y * F1X] = a2;
-            Tv[2 * F0Y * F1X] = pre;
-        }
-        pre = load_triplet(i + 1);
-        __syncthreads();
-        const int q = i - 1;  // h1 plane q from input planes q-1, q, q+1 (window planes 0, 1, 2)
-        if (q >= z0 - 2) {
-            const bool zin = q >= 0 && q < nz;
-#pragma unroll
-            for (int k = 0; k < G1W; ++k) {
-                if (w + FWAVES * k < F1G) {
-                    const int o16 = 16 * FWAVES * k;
-                    const u32x2 ta = TA[o16], tb = TB[o16], tc = TC[o16];
-                    u32x4 raw;
-                    raw[0] = ta[0];
-                    raw[1] = (ta[1] & 0xffffu) | (tb[0] << 16);
-                    raw[2] = (tb[0] >> 16) | (tb[1] << 16);
-                    raw[3] = c == 0 ? tc[0] : (c == 1 ? (tc[1] & 0xffffu) : 0u);
-                    const f32x4 acc = mfma16(A1, __builtin_bit_cast(bf16x8, raw), f32x4{0.f, 0.f, 0.f, 0.f});
-                    u32x2 pk = relu_bf16x4(acc);
-                    if (!(zin && ((xy1 >> k) & 1u))) pk[0] = pk[1] = 0u;
-                    *(u32x2 *)(H1w + o16 * 16) = pk;
-                }
-            }
-        }
-        __syncthreads();
-        // layer 2: h1 plane q feeds h2 planes q+1 (dz = -1), q (dz = 0) and q-1 (dz = +1, done)
-        if (q >= z0 - 2) {
-            const int pd = q - 1;
+        if (trip) T[(i + 1) & 3][threadIdx.x] = to_triplet(i + 1, pre0);
+        pre0 = pre1;
+        pre1 = pre2;
+        pre2 = fetch(i + 4);
+        // layer 2 first (most MFMA work; layers 1 and 3 fill its gaps):
+        // h1 plane q2 = i-2 feeds h2 planes q2+1 (NXT), q2 (CUR) and q2-1 (PRV, done)
+        const int q2 = i - 2;
+        if (q2 >= z0 - 2 && q2 <= z1 + 1) {
+            const int pd = q2 - 1;
             const bool keep = pd >= z0 - 1, zin = pd >= 0 && pd < nz;
+            const uint16_t *Hq = H1[q2 & 1];
+            uint16_t *Hw = H2[pd & 1] + v1 * VS + 4 * c;
 #pragma unroll
             for (int k = 0; k < G2W; ++k) {
                 if (w + FWAVES * k < F2G) {
                     u32x4 raw[NSTEP3F];
-                    load_pairs(H1, std::integral_constant<int, F1X>{}, F1V * 16, b2[k], raw);
+                    load_pairs(Hq, std::integral_constant<int, F1X>{}, F1V * VS, b2[k], raw);
                     f32x4 an = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
                     for (int p = 0; p < NSTEP3F; ++p) {
@@ -539,34 +525,64 @@ __global__ __launch_bounds__(FTHREADS) __attribute__((amdgpu_waves_per_eu(2, 2))
                     }
                     u32x2 pk = relu_bf16x4(acc2[PRV][k]);
                     acc2[NXT][k] = an;
-                    if (!(zin && ((xy2 >> k) & 1u))) pk[0] = pk[1] = 0u;
-                    if (keep && 16 * (w + FWAVES * k) + n < F2V) *(u32x2 *)(H2w + 16 * FWAVES * k * 16) = pk;
+                    if (!(zin && ((ok2 >> k) & 1u))) pk[0] = pk[1] = 0u;
+                    if (keep && 16 * (w + FWAVES * k) + n < F2V) *(u32x2 *)(Hw + 16 * FWAVES * k * VS) = pk;
                 }
             }
         }
-        __syncthreads();
-        // layer 3: h2 plane pp feeds output planes pp+1 (row 0), pp (row 1), pp-1 (row 2, done)
-        const int pp = i - 2;
-        if (pp >= z0 - 1) {
-            const int o = pp - 1;
-            const bool store = o >= z0 && o < z1;
+        // layer 1: h1 plane q1 = i-1 from input planes q1-1, q1, q1+1
+        const int q1 = i - 1;
+        if (q1 >= z0 - 2 && q1 <= z1 + 1) {
+            const bool zin = q1 >= 0 && q1 < nz;
+            const u32x2 *TA = T[(q1 - 1 + pA) & 3] + rA, *TB = T[(q1 - 1 + pB) & 3] + rB, *TC = T[(q1 + 1) & 3] + rC;
+            uint16_t *Hw = H1[q1 & 1] + v1 * VS + 4 * c;
 #pragma unroll
-            for (int k = 0; k < G3W; ++k) {
-                u32x4 raw[NSTEP3F];
-                load_pairs(H2, std::integral_constant<int, F2X>{}, F2V * 16, b3 + FWAVES * k * F2X * 16, raw);
-                f32x4 d = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-                for (int p = 0; p < NSTEP3F; ++p) d = mfma16(A3[p], __builtin_bit_cast(bf16x8, raw[p]), d);
-                acc3[NXT][k] = d[0];
-                acc3[CUR][k] = acc3[CUR][k] + d[1];
-                acc3[PRV][k] = acc3[PRV][k] + d[2];
-                if (store && ((xy3 >> k) & 1u))
-                    out[o * plane + ob + (int64_t)FWAVES * k * nx] = acc3[PRV][k] * oscale;
+            for (int k = 0; k < G1W; ++k) {
+                if (w + FWAVES * k < F1G) {
+                    const int o16 = 16 * FWAVES * k;
+                    const u32x2 ta = TA[o16], tb = TB[o16], tc = TC[o16];
+                    u32x4 raw;
+                    raw[0] = ta[0];
+                    raw[1] = (ta[1] & 0xffffu) | (tb[0] << 16);
+                    raw[2] = (tb[0] >> 16) | (tb[1] << 16);
+                    raw[3] = c == 0 ? tc[0] : (c == 1 ? (tc[1] & 0xffffu) : 0u);
+                    const f32x4 a = mfma16(A1, __builtin_bit_cast(bf16x8, raw), f32x4{0.f, 0.f, 0.f, 0.f});
+                    u32x2 pk = relu_bf16x4(a);
+                    if (!(zin && ((ok1 >> k) & 1u))) pk[0] = pk[1] = 0u;
+                    *(u32x2 *)(Hw + o16 * VS) = pk;
+                }
             }
         }
+        // layer 3: h2 plane q3 = i-4 feeds output planes q3+1 (row 0), q3 (row 1), q3-1 (row 2, done)
+        const int q3 = i - 4;
+        if (q3 >= z0 - 1 && q3 <= z1) {
+            const int o = q3 - 1;
+            const bool store = o >= z0 && o < z1;
+            const uint16_t *Hp = H2[q3 & 1];
+            u32x4 raw[G3W][NSTEP3F];
+#pragma unroll
+            for (int k = 0; k < G3W; ++k)
+                load_pairs(Hp, std::integral_constant<int, F2X>{}, F2V * VS, b3 + FWAVES * k * F2X * VS, raw[k]);
+            f32x4 d[G3W];
+#pragma unroll
+            for (int k = 0; k < G3W; ++k) d[k] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int p = 0; p < NSTEP3F; ++p)
+#pragma unroll
+                for (int k = 0; k < G3W; ++k) d[k] = mfma16(A3[p], __builtin_bit_cast(bf16x8, raw[k][p]), d[k]);
+#pragma unroll
+            for (int k = 0; k < G3W; ++k) {
+                acc3[NXT][k] = d[k][0];
+                acc3[CUR][k] = acc3[CUR][k] + d[k][1];
+                acc3[PRV][k] = acc3[PRV][k] + d[k][2];
+                if (store && ((ok3 >> k) & 1u)) out[o * plane + ob + (int64_t)FWAVES * k * nx] = acc3[PRV][k] * oscale;
+            }
+        }
+        __syncthreads();
     };
-    const int last = z1 + 2;
-    for (int i = z0 - 3; i <= last; i += 3) {
+    // steps z0-4 .. z1+4, unrolled by three for the rotating slots
+    const int last = z1 + 4;
+    for (int i = z0 - 4; i <= last; i += 3) {
         step(Slot<0>{}, i);
         if (i + 1 <= last) step(Slot<1>{}, i + 1);
         if (i + 2 <= last) step(Slot<2>{}, i + 2);
