@@ -365,10 +365,10 @@ constexpr int F1X = FX + 4, F1Y = FY + 4;   // h1 footprint (400 voxels, 25 grou
 constexpr int F0X = FX + 6, F0Y = FY + 6;   // input footprint (484 voxels)
 constexpr int F2V = F2X * F2Y, F1V = F1X * F1Y, F0V = F0X * F0Y;
 constexpr int F2G = (F2V + 15) / 16, F1G = F1V / 16, F3G = FX * FY / 16;
-constexpr int FWAVES = 8, FTHREADS = 64 * FWAVES;  // waves per fused workgroup
-constexpr int G1W = (F1G + FWAVES - 1) / FWAVES;   // voxel groups per wave: layer 1 (4),
-constexpr int G2W = (F2G + FWAVES - 1) / FWAVES;   //   layer 2 (3),
-constexpr int G3W = F3G / FWAVES;                  //   layer 3 (2)
+constexpr int FWAVES = 12, FTHREADS = 64 * FWAVES;  // waves per fused workgroup (3 per SIMD)
+constexpr int G1W = (F1G + FWAVES - 1) / FWAVES;    // voxel groups per wave: layer 1 (at most 3),
+constexpr int G2W = (F2G + FWAVES - 1) / FWAVES;    //   layer 2 (2),
+constexpr int G3W = (F3G + FWAVES - 1) / FWAVES;    //   layer 3 (2)
 constexpr int TPL = F0Y * F1X;                     // x-triplets per input-window plane (440)
 constexpr int VS = 16;  // h1 / h2 voxel record in LDS: 16 BF16 channels (32 bytes)
 constexpr int H1S = F1V * VS + 8, H2S = F2V * VS + 8;  // plane buffers + 8 zero channels
@@ -376,7 +376,7 @@ constexpr int H1S = F1V * VS + 8, H2S = F2V * VS + 8;  // plane buffers + 8 zero
 template <int K>
 using Slot = std::integral_constant<int, K>;
 
-__global__ __launch_bounds__(FTHREADS) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_fused_fwd(
+__global__ __launch_bounds__(FTHREADS) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_fused_fwd(
     const float *__restrict__ in, const float *__restrict__ in_scale, const bf16x8 *__restrict__ pk1,
     const bf16x8 *__restrict__ pk2, const bf16x8 *__restrict__ pk3f, float *__restrict__ out,
     const float *__restrict__ out_scale, int nx, int ny, int nz, int tiles_x, int tiles_y, int zchunk,
@@ -401,12 +401,13 @@ __global__ __launch_bounds__(FTHREADS) __attribute__((amdgpu_waves_per_eu(2, 2))
     const int half = (c & 1) * 8;  // channel half
     const int hiv = hi ? VS : 0;
     const int64_t plane = (int64_t)nx * ny;
+    // layer-1 and layer-2 weights stay in registers; layer 3's (read five times per step) in LDS
     const bf16x8 A1 = pk1[lane];
-    bf16x8 A2[NSTEP16], A3[NSTEP3F];
+    bf16x8 A2[NSTEP16];
 #pragma unroll
     for (int s = 0; s < NSTEP16; ++s) A2[s] = pk2[s * 64 + lane];
-#pragma unroll
-    for (int s = 0; s < NSTEP3F; ++s) A3[s] = pk3f[s * 64 + lane];
+    __shared__ bf16x8 A3s[NSTEP3F * 64];
+    for (int t = threadIdx.x; t < NSTEP3F * 64; t += FTHREADS) A3s[t] = pk3f[t];
     if (threadIdx.x < 8) {
         H1[0][F1V * VS + threadIdx.x] = 0;
         H1[1][F1V * VS + threadIdx.x] = 0;
@@ -444,7 +445,7 @@ __global__ __launch_bounds__(FTHREADS) __attribute__((amdgpu_waves_per_eu(2, 2))
     unsigned ok3 = 0;
 #pragma unroll
     for (int k = 0; k < G3W; ++k)
-        if (c == 0 && ox + n < nx && oy + w + FWAVES * k < ny) ok3 |= 1u << k;
+        if (w + FWAVES * k < F3G && c == 0 && ox + n < nx && oy + w + FWAVES * k < ny) ok3 |= 1u << k;
 
     // input loader: thread t < TPL owns triplet t (window row t / F1X, columns t % F1X .. +2).
     // The three loads read clamped, always valid addresses without branches, so they stay in
@@ -456,11 +457,15 @@ __global__ __launch_bounds__(FTHREADS) __attribute__((amdgpu_waves_per_eu(2, 2))
     const bool tx0 = tgx0 >= 0 && tgx0 < nx, tx1 = tgx0 + 1 >= 0 && tgx0 + 1 < nx, tx2 = tgx0 + 2 >= 0 && tgx0 + 2 < nx;
     const float *trowp = in + (int64_t)min(max(tgy, 0), ny - 1) * nx;
     const int cx0 = min(max(tgx0, 0), nx - 1), cx1 = min(max(tgx0 + 1, 0), nx - 1), cx2 = min(max(tgx0 + 2, 0), nx - 1);
-    auto fetch = [&](int ip) -> f32x4 {
-        const float *row = trowp + (int64_t)min(max(ip, 0), nz - 1) * plane;
-        return f32x4{row[cx0], row[cx1], row[cx2], 0.f};
+    struct F3 {
+        float v[3];
     };
-    auto to_triplet = [&](int ip, f32x4 f) -> u32x2 {
+    auto fetch = [&](int ip) -> F3 {
+        const float *row = trowp + (int64_t)min(max(ip, 0), nz - 1) * plane;
+        return F3{{row[cx0], row[cx1], row[cx2]}};
+    };
+    auto to_triplet = [&](int ip, F3 g) -> u32x2 {
+        const float *f = g.v;
         const bool ok = trow && ip >= 0 && ip < nz;
         u32x2 t3;
         t3[0] = pack_bf16(ok && tx0 ? f[0] * scale : 0.f, ok && tx1 ? f[1] * scale : 0.f);
@@ -492,7 +497,7 @@ __global__ __launch_bounds__(FTHREADS) __attribute__((amdgpu_waves_per_eu(2, 2))
     }
     // input planes are requested three steps before they are stored (step i stores plane i+1,
     // loaded during step i-3), so the global latency hides behind three steps of work
-    f32x4 pre0 = fetch(z0 - 3), pre1 = fetch(z0 - 2), pre2 = fetch(z0 - 1);
+    F3 pre0 = fetch(z0 - 3), pre1 = fetch(z0 - 2), pre2 = fetch(z0 - 1);
 
     // step i; rotating slot R holds the partial sums the step starts (h2 plane i-1, output plane
     // i-3), R+2 those it continues, R+1 those it completes
@@ -559,23 +564,19 @@ __global__ __launch_bounds__(FTHREADS) __attribute__((amdgpu_waves_per_eu(2, 2))
             const int o = q3 - 1;
             const bool store = o >= z0 && o < z1;
             const uint16_t *Hp = H2[q3 & 1];
-            u32x4 raw[G3W][NSTEP3F];
-#pragma unroll
-            for (int k = 0; k < G3W; ++k)
-                load_pairs(Hp, std::integral_constant<int, F2X>{}, F2V * VS, b3 + FWAVES * k * F2X * VS, raw[k]);
-            f32x4 d[G3W];
-#pragma unroll
-            for (int k = 0; k < G3W; ++k) d[k] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int p = 0; p < NSTEP3F; ++p)
-#pragma unroll
-                for (int k = 0; k < G3W; ++k) d[k] = mfma16(A3[p], __builtin_bit_cast(bf16x8, raw[k][p]), d[k]);
 #pragma unroll
             for (int k = 0; k < G3W; ++k) {
-                acc3[NXT][k] = d[k][0];
-                acc3[CUR][k] = acc3[CUR][k] + d[k][1];
-                acc3[PRV][k] = acc3[PRV][k] + d[k][2];
-                if (store && ((ok3 >> k) & 1u)) out[o * plane + ob + (int64_t)FWAVES * k * nx] = acc3[PRV][k] * oscale;
+                if (w + FWAVES * k < F3G) {
+                    u32x4 raw[NSTEP3F];
+                    load_pairs(Hp, std::integral_constant<int, F2X>{}, F2V * VS, b3 + FWAVES * k * F2X * VS, raw);
+                    f32x4 d = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                    for (int p = 0; p < NSTEP3F; ++p) d = mfma16(A3s[p * 64 + lane], __builtin_bit_cast(bf16x8, raw[p]), d);
+                    acc3[NXT][k] = d[0];
+                    acc3[CUR][k] = acc3[CUR][k] + d[1];
+                    acc3[PRV][k] = acc3[PRV][k] + d[2];
+                    if (store && ((ok3 >> k) & 1u)) out[o * plane + ob + (int64_t)FWAVES * k * nx] = acc3[PRV][k] * oscale;
+                }
             }
         }
         __syncthreads();
