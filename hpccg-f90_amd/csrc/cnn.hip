@@ -444,22 +444,22 @@ __device__ __forceinline__ bool kf_true_due(const FcgState *st)
 }
 
 template <bool PREP>
-__global__ __launch_bounds__(kSellThreads) void kf_true(const SellRef<__half> A, const double *__restrict__ x,
+__global__ __launch_bounds__(kTrueThreads) void kf_true(const SellRef<__half> A, const double *__restrict__ x,
                                                         const double *__restrict__ b, float *__restrict__ r,
                                                         const FcgState *st, double *red_part)
 {
-    __shared__ double red[kSellThreads / kWave];
+    __shared__ double red[kTrueThreads / kWave];
     if (!kf_true_due<PREP>(st)) return;
     int64_t row;
     bool valid;
-    const double ax = sell_rowsum_of_thread<kGroupF16>(A, x, row, valid);
+    const double ax = sell_rowsum_of_thread<kGroupF16, __half, double, kTrueWaves>(A, x, row, valid);
     double dot = 0.0;
     if (valid) {
         const float ri = (float)(b[row] - ax);
         r[row] = ri;
         dot = (double)ri * (double)ri;
     }
-    store_partial(block_sum<kSellThreads>(dot, red), red_part);
+    store_partial(block_sum<kTrueThreads>(dot, red), red_part);
 }
 
 // r.r of the true residual: network scales, stopping test; PREP also decides the first stop
@@ -1077,7 +1077,7 @@ int hpccg_fcg_learned_dev(hpccg_ctx *c, int32_t nx, int32_t ny, int32_t nz, cons
     HPCCG_HIP_TRY(hipMemcpyAsync(F->hist, neg.data(), neg.size() * 8, hipMemcpyHostToDevice, c->stream));
     double *hrec = F->hist, *htrue = F->hist + F->hist_cap;
     const Matrix &A = c->A;
-    const int Gs = sell_grid(A), Gv = vec_grid_for(c->info.num_cu, n);
+    const int Gs = sell_grid(A), Gt = sell_grid(A, kTrueWaves), Gv = vec_grid_for(c->info.num_cu, n);
     const int *skip = &F->st->done;
     const float *s_in = &F->st->s_in, *s_out = &F->st->s_out;
 
@@ -1089,9 +1089,9 @@ int hpccg_fcg_learned_dev(hpccg_ctx *c, int32_t nx, int32_t ny, int32_t nz, cons
                            c->partials);
         hipLaunchKernelGGL(kf_xr_finish, dim3(1), dim3(kFinishThreads), 0, c->stream, c->partials, Gv, n, F->st,
                            hrec);
-        hipLaunchKernelGGL(kf_true<false>, dim3(Gs), dim3(kSellThreads), 0, c->stream, sell16(A), x, b, F->r, F->st,
+        hipLaunchKernelGGL(kf_true<false>, dim3(Gt), dim3(kTrueThreads), 0, c->stream, sell16(A), x, b, F->r, F->st,
                            A.red_part);
-        hipLaunchKernelGGL(kf_true_finish<false>, dim3(1), dim3(kFinishThreads), 0, c->stream, A.red_part, Gs, n,
+        hipLaunchKernelGGL(kf_true_finish<false>, dim3(1), dim3(kFinishThreads), 0, c->stream, A.red_part, Gt, n,
                            F->st, htrue);
         forward_fused(c, F->r, s_in, F->z, s_out, g, skip);
         hipLaunchKernelGGL(kf_dots<false>, dim3(Gv), dim3(kVecThreads), 0, c->stream, F->z, F->Ap, F->r, n, F->st,
@@ -1102,9 +1102,9 @@ int hpccg_fcg_learned_dev(hpccg_ctx *c, int32_t nx, int32_t ny, int32_t nz, cons
 
     HPCCG_HIP_TRY(hipEventRecord(c->ev0, c->stream));
     // Prepare: r = b - A x, z = M(r), rz = z.r, p = z
-    hipLaunchKernelGGL(kf_true<true>, dim3(Gs), dim3(kSellThreads), 0, c->stream, sell16(A), x, b, F->r, F->st,
+    hipLaunchKernelGGL(kf_true<true>, dim3(Gt), dim3(kTrueThreads), 0, c->stream, sell16(A), x, b, F->r, F->st,
                        A.red_part);
-    hipLaunchKernelGGL(kf_true_finish<true>, dim3(1), dim3(kFinishThreads), 0, c->stream, A.red_part, Gs, n, F->st,
+    hipLaunchKernelGGL(kf_true_finish<true>, dim3(1), dim3(kFinishThreads), 0, c->stream, A.red_part, Gt, n, F->st,
                        htrue);
     forward_fused(c, F->r, s_in, F->z, s_out, g, skip);
     hipLaunchKernelGGL(kf_dots<true>, dim3(Gv), dim3(kVecThreads), 0, c->stream, F->z, F->Ap, F->r, n, F->st,

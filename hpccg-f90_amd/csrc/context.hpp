@@ -55,7 +55,7 @@ inline SellRef<__half> sell16(const Matrix &A)
     return SellRef<__half>{A.slice_ptr, A.scol16, A.sval16, A.rowindex, A.nrow, A.nslices};
 }
 // workgroups of a one-wave-per-slice launch (at least one, so reductions always run)
-inline int sell_grid(const Matrix &A) { return A.nslices > 0 ? sell_blocks(A.nslices) : 1; }
+inline int sell_grid(const Matrix &A, int waves = kSellWaves) { return A.nslices > 0 ? sell_blocks(A.nslices, waves) : 1; }
 
 struct GraphKey {
     const void *b, *x;

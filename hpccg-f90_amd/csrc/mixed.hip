@@ -52,22 +52,22 @@ __device__ __forceinline__ bool km_true_due(const MixState *st)
 }
 
 template <bool PREP>
-__global__ __launch_bounds__(kSellThreads) void km_true_residual(const SellRef<__half> A, const double *__restrict__ x,
+__global__ __launch_bounds__(kTrueThreads) void km_true_residual(const SellRef<__half> A, const double *__restrict__ x,
                                                                  const double *__restrict__ b, float *__restrict__ r,
                                                                  const MixState *st, double *red_part)
 {
-    __shared__ double red[kSellThreads / kWave];
+    __shared__ double red[kTrueThreads / kWave];
     if (!km_true_due<PREP>(st)) return;
     int64_t row;
     bool valid;
-    const double ax = sell_rowsum_of_thread<kGroupF16>(A, x, row, valid);
+    const double ax = sell_rowsum_of_thread<kGroupF16, __half, double, kTrueWaves>(A, x, row, valid);
     double dot = 0.0;
     if (valid) {
         const float ri = (float)(b[row] - ax);
         r[row] = ri;
         dot = (double)ri * (double)ri;
     }
-    store_partial(block_sum<kSellThreads>(dot, red), red_part);
+    store_partial(block_sum<kTrueThreads>(dot, red), red_part);
 }
 
 // r.r of the true residual: replacement, convergence test on the FP64 true residual, beta.
@@ -302,10 +302,10 @@ template <bool PREP>
 static void launch_km_true(hpccg_ctx *c, const double *b, const double *x)
 {
     const Matrix &A = c->A;
-    const int G = sell_grid(A);
+    const int G = sell_grid(A, kTrueWaves);
     MixedWs &m = c->mix;
     MixState *st = (MixState *)m.state;
-    hipLaunchKernelGGL(km_true_residual<PREP>, dim3(G), dim3(kSellThreads), 0, c->stream, sell16(A), x, b, m.r, st,
+    hipLaunchKernelGGL(km_true_residual<PREP>, dim3(G), dim3(kTrueThreads), 0, c->stream, sell16(A), x, b, m.r, st,
                        A.red_part);
     hipLaunchKernelGGL(km_true_finish<PREP>, dim3(1), dim3(kFinishThreads), 0, c->stream, A.red_part, G, st,
                        m.hist + m.hist_cap);

@@ -30,6 +30,10 @@ namespace hpccg {
 constexpr int kSlice = 64;            // rows per slice == lanes per wave
 constexpr int kSellWaves = 4;         // slices per SpMV workgroup (one per wave)
 constexpr int kSellThreads = kSlice * kSellWaves;
+// true-residual kernels run only every few iterations and exit at once otherwise: 16 slices
+// per workgroup keep the cost of those empty launches low (16x fewer workgroups)
+constexpr int kTrueWaves = 16;
+constexpr int kTrueThreads = kSlice * kTrueWaves;
 constexpr int kStencilK = 27;         // slice width of the 27-point stencil: fully unrolled path
 constexpr int kGroupF64 = 2;          // FP64 copy: 2 entries per lane per load (16-byte value loads)
 constexpr int kGroupF16 = 1;          // FP16 copy: 1 entry per lane per load (measured best)
@@ -144,13 +148,13 @@ struct SellRef {
     int nrow, nslices;
 };
 
-// One wave per slice, kSellWaves slices per workgroup: the FP64 row sum of this thread's row
-// (0 past the last row or slice), with the row and whether it exists.
-template <int G, typename VT, typename XT>
+// One wave per slice, WAVES slices per workgroup: the FP64 row sum of this thread's row (0 past
+// the last row or slice), with the row and whether it exists.
+template <int G, typename VT, typename XT, int WAVES = kSellWaves>
 __device__ __forceinline__ double sell_rowsum_of_thread(const SellRef<VT> &A, const XT *__restrict__ x, int64_t &row,
                                                         bool &valid)
 {
-    const int s = blockIdx.x * kSellWaves + __builtin_amdgcn_readfirstlane(threadIdx.x / kSlice);
+    const int s = blockIdx.x * WAVES + __builtin_amdgcn_readfirstlane(threadIdx.x / kSlice);
     const int lane = threadIdx.x % kSlice;
     row = (int64_t)s * kSlice + lane;
     valid = s < A.nslices && row < A.nrow;
@@ -223,12 +227,28 @@ __device__ __forceinline__ void store_partial(double block_total, double *part)
 }
 
 // Sum of part[0..n) by the single workgroup of a finish kernel (kFinishThreads threads); every
-// thread gets the total.  `red` needs kFinishThreads/64 doubles of LDS.
+// thread gets the total.  `red` needs kFinishThreads/64 doubles of LDS.  Each thread sums
+// its strided share with eight independent loads in flight (a 65,536-partial sum is bound by
+// one CU's load latency otherwise), in a fixed order.
 __device__ __forceinline__ double sum_partials(const double *__restrict__ part, int n, double *red)
 {
-    double s = 0.0;
-    for (int i = threadIdx.x; i < n; i += kFinishThreads) s += part[i];
-    return block_sum<kFinishThreads>(s, red);
+    constexpr int U = 8;
+    double s[U] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    int i = threadIdx.x;
+    for (; i + (U - 1) * kFinishThreads < n; i += U * kFinishThreads) {
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = part[i + u * kFinishThreads];
+#pragma unroll
+        for (int u = 0; u < U; ++u) s[u] += v[u];
+    }
+#pragma unroll
+    for (int u = 0; u < U - 1; ++u)
+        if (i + u * kFinishThreads < n) s[u] += part[i + u * kFinishThreads];
+    double t = 0.0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) t += s[u];
+    return block_sum<kFinishThreads>(t, red);
 }
 
 // Element range [i0, i1) of one (XCD-renumbered) workgroup in a contiguous split of n.
@@ -243,7 +263,7 @@ __device__ __forceinline__ void elem_range(int64_t n, int64_t &i0, int64_t &i1, 
 // Launch shapes (host).  SpMV kernels launch one wave per slice, kSellWaves slices per
 // workgroup, in slice order (the hardware dispatches workgroups in order, so the waves in
 // flight stream a compact window of the matrix); the vector kernels use a resident grid.
-inline int sell_blocks(int64_t nslices) { return (int)((nslices + kSellWaves - 1) / kSellWaves); }
+inline int sell_blocks(int64_t nslices, int waves = kSellWaves) { return (int)((nslices + waves - 1) / waves); }
 inline int vec_grid_for(int num_cu, int64_t n)
 {
     int64_t G = (n + kVecThreads - 1) / kVecThreads;
