@@ -188,7 +188,7 @@ __global__ __launch_bounds__(ROWS) void kb_true(const BatchSell A, const double 
     team_rows<ROWS>(A, tp, x, [&](int64_t gr, double ax) {
         const double bi = b[gr];
         const float ri = (float)(bi - ax);
-        r[gr] = ri;
+        __builtin_nontemporal_store(ri, r + gr);
         rr = fma((double)ri, (double)ri, rr);
         if (PREP) bb = fma(bi, bi, bb);
     });
@@ -274,7 +274,7 @@ __global__ __launch_bounds__(ROWS) void kb_spmv(const BatchSell A, const float *
     double dot = 0.0;
     team_rows<ROWS>(A, tp, p, [&](int64_t gr, double s) {
         const float a = (float)s;
-        Ap[gr] = a;
+        __builtin_nontemporal_store(a, Ap + gr);
         dot = fma((double)p[gr], (double)a, dot);
     });
     dot = block_sum<ROWS>(dot, red);

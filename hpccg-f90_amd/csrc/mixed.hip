@@ -64,7 +64,7 @@ __global__ __launch_bounds__(kTrueThreads) void km_true_residual(const SellRef<_
     double dot = 0.0;
     if (valid) {
         const float ri = (float)(b[row] - ax);
-        r[row] = ri;
+        __builtin_nontemporal_store(ri, r + row);
         dot = (double)ri * (double)ri;
     }
     store_partial(block_sum<kTrueThreads>(dot, red), red_part);
@@ -131,7 +131,7 @@ __global__ __launch_bounds__(kSellThreads) void km_spmv(const SellRef<__half> A,
     double dot = 0.0;
     if (valid) {
         const float a = (float)s;
-        Ap[row] = a;
+        __builtin_nontemporal_store(a, Ap + row);
         dot = (double)p[row] * (double)a;
     }
     store_partial(block_sum<kSellThreads>(dot, red), red_part);

@@ -11,7 +11,9 @@
 //     (K_s / G) * G entries and at  base + KG * 64 + (j - KG) * 64 + l  after them (KG =
 //     (K_s / G) * G), so that one load instruction of the wave reads 64 * G consecutive
 //     entries (fully coalesced) and each lane gets G consecutive entries of its own row;
-//   * columns are stored 0-based; slots past a row's end hold value 0 and a valid column.
+//   * columns are stored 0-based; slots past a row's end hold value 0 and a valid column;
+//   * slot offsets are multiples of 64, so bit 0 of slice_ptr[s] is free: it flags a "full"
+//     slice (64 rows, all exactly K_s long), whose row lengths need no rowindex reads.
 //
 // Each lane sums its own row in ascending entry order from 0.0 and skips the padding slots
 // (the row length comes from rowindex), so y = A x is bitwise the reference's row loop
@@ -96,6 +98,9 @@ __host__ __device__ __forceinline__ int64_t sell_slot(int64_t base, int K, int j
     return base + (int64_t)KG * kSlice + (int64_t)(j - KG) * kSlice + lane;
 }
 
+constexpr int64_t kSliceFull = 1;                       // flag bit of slice_ptr[s]
+__host__ __device__ __forceinline__ int64_t slice_base(int64_t sp) { return sp & ~(int64_t)(kSlice - 1); }
+
 // Row sum (FP64, ascending entry order from 0.0) of global row `row` held by lane `lane` of
 // slice `s`; 0 when !valid.  rowptr is the 1-based CSR row pointer (row lengths).
 template <int G, typename VT, typename XT>
@@ -106,9 +111,9 @@ __device__ __forceinline__ double sell_rowsum(const int64_t *__restrict__ slice_
                                               const XT *__restrict__ x, int s, int lane, int64_t row,
                                               bool valid)
 {
-    const int len = valid ? rowptr[row + 1] - rowptr[row] : 0;
-    const int64_t base = slice_ptr[s];
-    const int K = (int)((slice_ptr[s + 1] - base) / kSlice);
+    const int64_t sp0 = slice_ptr[s], base = slice_base(sp0);
+    const int K = (int)((slice_base(slice_ptr[s + 1]) - base) / kSlice);
+    const int len = !valid ? 0 : (sp0 & kSliceFull) ? K : rowptr[row + 1] - rowptr[row];
     double sum = 0.0;
     if (K == kStencilK) {
         // branch-free: all column and value loads issue back to back, then the gathers
@@ -174,8 +179,8 @@ __global__ void k_csr_to_sell(const int32_t *__restrict__ rowptr, const int32_t 
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (t >= (int64_t)nslices * kSlice) return;
     const int s = (int)(t / kSlice), lane = (int)(t % kSlice);
-    const int64_t base = slice_ptr[s];
-    const int K = (int)((slice_ptr[s + 1] - base) / kSlice);
+    const int64_t base = slice_base(slice_ptr[s]);
+    const int K = (int)((slice_base(slice_ptr[s + 1]) - base) / kSlice);
     int len = 0, padc = 0, bad = 0;
     if (t < nrow) {
         const int64_t k0 = rowptr[t] - 1;
@@ -198,17 +203,23 @@ __global__ void k_csr_to_sell(const int32_t *__restrict__ rowptr, const int32_t 
     if (bad && inexact) atomicOr(inexact, 1);
 }
 
-// Host: slot offsets of the slices of rows [0, nrow) of a 1-based CSR row pointer; returns
-// the total slot count.  sp must hold ceil(nrow/64)+1 entries; sp[0] = base.
+// Host: slot offsets (with the full-slice flags) of the slices of rows [0, nrow) of a 1-based
+// CSR row pointer; returns the total slot count.  sp must hold ceil(nrow/64)+1 entries and
+// sp[0] = base (a multiple of 64) on entry; sp[ns] carries no flag.
 inline int64_t sell_slice_ptr(const int32_t *rowptr, int64_t nrow, int64_t base, int64_t *sp)
 {
     const int64_t ns = (nrow + kSlice - 1) / kSlice;
     sp[0] = base;
     for (int64_t s = 0; s < ns; ++s) {
-        int K = 0;
+        int K = 0, kmin = 1 << 30;
         const int64_t r1 = nrow < (s + 1) * kSlice ? nrow : (s + 1) * kSlice;
-        for (int64_t r = s * kSlice; r < r1; ++r) K = K > rowptr[r + 1] - rowptr[r] ? K : rowptr[r + 1] - rowptr[r];
-        sp[s + 1] = sp[s] + (int64_t)K * kSlice;
+        for (int64_t r = s * kSlice; r < r1; ++r) {
+            const int len = rowptr[r + 1] - rowptr[r];
+            K = K > len ? K : len;
+            kmin = kmin < len ? kmin : len;
+        }
+        sp[s + 1] = slice_base(sp[s]) + (int64_t)K * kSlice;
+        if (r1 - s * kSlice == kSlice && kmin == K) sp[s] |= kSliceFull;
     }
     return sp[ns] - base;
 }
