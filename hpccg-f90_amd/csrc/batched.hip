@@ -311,7 +311,8 @@ __global__ __launch_bounds__(ROWS) void kb_update_xr(double *__restrict__ x, flo
     double rr;
     if (grid_sum<ROWS>(dot, partials + (lb - tp.l), &S->counter[2], tp.g, tp.l, red, &rr) && threadIdx.x == 0) {
         const int k = S->k;
-        const bool check = (S->replace_every > 0 && k % S->replace_every == 0) || rr < S->tol2 || k >= S->max_iter;
+        const bool check = (S->replace_every > 0 && k % S->replace_every == 0) || rr < S->tol2 ||
+                           k >= S->max_iter || rr < kReplaceDrop2 * S->rr_true;
         if (check) {
             S->need_true = 1;
         } else {

@@ -564,7 +564,7 @@ __global__ __launch_bounds__(kFinishThreads) void kf_xr_finish(const double *__r
         st->s_in = rms > 0.0 ? (float)(1.0 / rms) : 1.0f;
         st->s_out = (float)rms;
         const bool check = (st->replace_every > 0 && k % st->replace_every == 0) || sqrt(rr) < st->tol ||
-                           k >= st->max_iter;
+                           k >= st->max_iter || rr < kReplaceDrop2 * st->rr_true;
         if (check) st->need_true = 1;
         else st->k = k + 1;
         st->done = st->k >= st->stop_at;
@@ -868,8 +868,15 @@ int hpccg_cnn_make_dataset(hpccg_ctx *c, int32_t n, int32_t npairs, int32_t crop
         st = cg_trace(c, b, x, 400, 1e-10 * std::sqrt(bb), [&](int k) {
             (void)k;
             if (D.count >= npairs) return 1;
-            const int ox = (int)(rnd() % (uint64_t)(n - crop + 1)), oy = (int)(rnd() % (uint64_t)(n - crop + 1)),
-                      oz = (int)(rnd() % (uint64_t)(n - crop + 1));
+            // window origin per axis: against the low domain face, against the high one, or
+            // uniform, one third each.  Rows on domain faces, edges and corners have 18, 12 or 8
+            // entries; uniform windows would touch them rarely (2 in 49 per axis at 64^3, crop 16)
+            // and leave the network untrained -- and its preconditioner indefinite -- there.
+            auto origin = [&]() {
+                const uint64_t u = rnd() % 3u;
+                return u == 0 ? 0 : u == 1 ? n - crop : (int)(rnd() % (uint64_t)(n - crop + 1));
+            };
+            const int ox = origin(), oy = origin(), oz = origin();
             hipLaunchKernelGGL(k_crop, dim3(1), dim3(256), 0, c->stream, c->r, x, c->p, xs, c->state, n, crop, ox,
                                oy, oz, D.r + D.count * V, D.e + D.count * V);
             // loss-mask margins: 4 on crop faces inside the domain, 0 on domain faces

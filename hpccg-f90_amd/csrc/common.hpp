@@ -41,6 +41,13 @@ void set_error(const char *fmt, ...);
 constexpr int kWave = 64;
 constexpr int kNumXcd = 8;  // MI355X: 8 XCDs, each with its own L2; workgroup i -> XCD i % 8
 
+// Mixed-precision solvers keep r in FP32: the recursive residual drifts from b - A x by about
+// 1e-7 of the residual norm at the last true-residual replacement, so it cannot show a 1e-10
+// reduction by itself.  Besides the periodic replacement, the FP64 true residual is formed
+// again whenever the recursive r.r has fallen below this fraction of the last true r.r (a
+// 1e5-fold drop in the norm; each true residual lowers the drift floor by the same factor).
+constexpr double kReplaceDrop2 = 1e-10;
+
 // Workgroups are dealt to XCDs round-robin.  Renumber them so that XCD x owns the contiguous
 // logical range [x*G/8, (x+1)*G/8): neighbouring row tiles (which share x-vector planes)
 // then run on the same XCD and hit in that XCD's L2.  G must be a multiple of 8.

@@ -30,6 +30,7 @@ struct MixState {
     double beta;
     double rtrans;  // r.r used by the current iteration (recursive or replaced)
     double tol;
+    double rr_true;  // r.r of the last true residual
     int k;
     int stop_at;
     int max_iter;
@@ -81,6 +82,7 @@ __global__ __launch_bounds__(kFinishThreads) void km_true_finish(const double *_
     const double rr = sum_partials(red_part, nparts, red);
     if (threadIdx.x == 0) {
         hist_true[k] = rr;
+        st->rr_true = rr;
         const bool conv = sqrt(rr) < st->tol;
         if (PREP) {
             st->rtrans = rr;
@@ -173,7 +175,8 @@ __global__ __launch_bounds__(kVecThreads) void km_update_xr(double *__restrict__
 }
 
 // recursive r.r; decides whether the true residual is formed next (every replace_every
-// iterations, when the recursive norm passes the tolerance, and at max_iter)
+// iterations, after a kReplaceDrop2 fall since the last one, when the recursive norm passes
+// the tolerance, and at max_iter)
 __global__ __launch_bounds__(kFinishThreads) void km_xr_finish(const double *__restrict__ partials, int nparts,
                                                                MixState *st, double *hist_rec)
 {
@@ -184,7 +187,7 @@ __global__ __launch_bounds__(kFinishThreads) void km_xr_finish(const double *__r
     if (threadIdx.x == 0) {
         hist_rec[k] = rr;
         const bool check = (st->replace_every > 0 && k % st->replace_every == 0) || sqrt(rr) < st->tol ||
-                           k >= st->max_iter;
+                           k >= st->max_iter || rr < kReplaceDrop2 * st->rr_true;
         if (check) {
             st->need_true = 1;
         } else {

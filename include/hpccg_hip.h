@@ -192,8 +192,9 @@ int hpccg_cnn_forward(hpccg_ctx *ctx, int32_t nx, int32_t ny, int32_t nz, int32_
 int hpccg_cnn_backward(hpccg_ctx *ctx, int32_t nx, int32_t ny, int32_t nz, int32_t batch,
                        const float *u, const float *dz, float *dW);
 /* Training set: npairs (r_k, e_k) pairs from plain FP64 CG runs on n^3 grids (seeded
- * x* = hpccg_xstar_sines), each a random crop^3 window scaled by 1/rms(r).  Replaces the
- * context's matrix. */
+ * x* = hpccg_xstar_sines), each a random crop^3 window scaled by 1/rms(r); per axis the window
+ * lies against the low domain face, the high one, or anywhere, with probability 1/3 each.
+ * Replaces the context's matrix. */
 int hpccg_cnn_make_dataset(hpccg_ctx *ctx, int32_t n, int32_t npairs, int32_t crop, uint64_t seed);
 /* Pair i of the training set (crop^3 floats each); margins6 (may be NULL) = loss-mask margins. */
 int hpccg_cnn_dataset_pair(hpccg_ctx *ctx, int32_t i, float *r, float *e, int32_t *margins6);

@@ -137,6 +137,14 @@ def test_dataset_pairs_satisfy_A_e_eq_r(ctx, trained):
         assert abs(np.sqrt(np.mean(r.astype(np.float64) ** 2)) - 1.0) < 1e-5  # scaled by 1/rms(r)
 
 
+def test_dataset_windows_cover_domain_faces(ctx, trained):
+    """Per axis a window lies against the low face, the high face or anywhere (1/3 each), so
+    about 1/3 + (1/3)(1/17) of the margins are 0 at 32^3 with 16^3 crops (17 origins)."""
+    zeros = [int(x) for i in range(0, 1500, 5) for x in ctx.cnn_dataset_pair(i, 16)[2] if x == 0]
+    frac = len(zeros) / (300 * 6)
+    assert 0.28 <= frac <= 0.42, frac
+
+
 def test_training_reduces_loss(trained):
     loss = trained["loss"]
     assert np.all(np.isfinite(loss))
@@ -151,7 +159,7 @@ def _system(n, seed):
 
 def test_fcg_learned_gain_and_accuracy(ctx):
     """The shipped network (hpccg_amd/data, trained on 64^3 pairs and selected on a 40^3
-    validation system) cuts flexible-CG iterations well below plain CG's on a fresh 32^3
+    validation system) at least halves flexible-CG iterations against plain CG's on a fresh 32^3
     system, and the answer meets the tolerance in the FP64 true residual (judged by the oracle)."""
     from hpccg_amd import learned
     w = learned.load()
@@ -169,7 +177,7 @@ def test_fcg_learned_gain_and_accuracy(ctx):
     assert rel <= 1e-10, (rel, res)
     _, nit_plain, _, _, _ = oracle.cg(A, b, max_iter=500, tolerance=1e-10 * normb)
     assert not res["fell_back"], res
-    assert res["fcg_iters"] <= 0.8 * nit_plain, (res["fcg_iters"], nit_plain)
+    assert res["fcg_iters"] <= 0.5 * nit_plain, (res["fcg_iters"], nit_plain)
 
 
 def test_training_pipeline_selects_a_network_that_beats_plain_cg(ctx):
