@@ -175,18 +175,16 @@ def main():
         d = b_h - rr
         return math.sqrt(float(d @ d)) / normb
 
+    # Each solve starts from x0 = 0, written into x (device) before the clock starts: the timed
+    # region is the solve on inputs already resident in HBM.
     def run_fp64():
-        x.upload(zeros)
-        res = ctx.cg_dev(b, x, max_iter=2000, tolerance=REL_TOL * normb, want_hist=False)
-        return res
+        return ctx.cg_dev(b, x, max_iter=2000, tolerance=REL_TOL * normb, want_hist=False)
 
     def run_mixed():
-        x.upload(zeros)
         return ctx.cg_mixed_dev(b, x, max_iter=2000, tolerance=REL_TOL * normb, replace_every=25,
                                 want_hist=False)
 
     def run_learned():
-        x.upload(zeros)
         r = ctx.fcg_learned_dev(grid, b, x, max_iter=2000, tolerance=REL_TOL * normb, replace_every=25)
         r["niters"] = r["fcg_iters"] + r["cg_iters"]
         return r
@@ -197,9 +195,11 @@ def main():
     results = {}
     for name, fn in solvers.items():
         for _ in range(args.warmup):
+            x.upload(zeros)
             fn()
         times, dev_ms = [], []
         for _ in range(args.steps):
+            x.upload(zeros)  # synchronous host-to-device copy: done before the clock starts
             t = time.perf_counter()
             res = fn()
             times.append(time.perf_counter() - t)

@@ -73,6 +73,7 @@ __device__ __forceinline__ void cg_after_pap(CgState *st, const double *rr_hist,
 __device__ __forceinline__ void cg_after_rr(CgState *st, double *rr_hist, int k, double rr)
 {
     rr_hist[k] = rr;
+    st->rr = rr;
     const double normr = sqrt(rr);
     if (k == 0) {
         if (normr < st->tol) st->stop_at = 1;
@@ -573,6 +574,10 @@ static int cg_device(hpccg_ctx *c, const double *b, double *x, int max_iter, dou
     *c->state_h = init;
     HPCCG_HIP_TRY(hipMemcpyAsync(c->state, c->state_h, sizeof(CgState), hipMemcpyHostToDevice, c->stream));
 
+    const int cap = tree ? 16 : 4;
+    auto get = [&](int size, hipGraphExec_t *ge) { return get_chunk_graph(c, b, x, size, ge); };
+    HPCCG_TRY(capture_iteration_graphs(cap, max_iter, get));
+
     HPCCG_HIP_TRY(hipEventRecord(c->ev0, c->stream));
     // Prepare (numerics.F90:92-99): Ap = A x; r = b - Ap; rtrans = r.r
     HPCCG_TRY(launch_spmv(c, x, c->Ap));
@@ -588,22 +593,25 @@ static int cg_device(hpccg_ctx *c, const double *b, double *x, int max_iter, dou
     }
     HPCCG_HIP_TRY(hipGetLastError());
 
-    // Iteration loop ("Total" timer, numerics.F90:106-150): graphs of `chunk` iterations.
-    const int chunk = std::max(1, std::min(max_iter, tree ? 16 : 4));
-    hipGraphExec_t ge = nullptr;
-    if (max_iter > 0) HPCCG_TRY(get_chunk_graph(c, b, x, chunk, &ge));
-    int launched = 0;
-    bool done = max_iter == 0;
+    // Iteration loop ("Total" timer, numerics.F90:106-150): graphs of up to 16 (tree) or 4
+    // (sequential dots) iterations.  With a tolerance the host reads the state after each
+    // launch and sizes the next one; the reference's lagging test runs one iteration after
+    // ||r|| < tol.  Without one, all max_iter iterations are enqueued at once.
     const bool can_stop_early = tol > 0.0 || std::isnan(tol);
-    while (!done) {
-        HPCCG_HIP_TRY(hipGraphLaunch(ge, c->stream));
-        launched += chunk;
+    ChunkPlanner plan{cap, tol, 1};
+    int launched = 0, m = std::min(cap, max_iter);
+    while (launched < max_iter) {
+        HPCCG_TRY(launch_iterations(c->stream, m, get));
+        launched += m;
         if (launched >= max_iter) break;
         if (can_stop_early) {
-            // read the state of the previous chunk while this one runs
             HPCCG_HIP_TRY(hipMemcpyAsync(c->state_h, c->state, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
             HPCCG_HIP_TRY(hipStreamSynchronize(c->stream));
-            done = c->state_h->k >= c->state_h->stop_at;
+            const CgState &sh = *c->state_h;
+            if (sh.k >= sh.stop_at) break;
+            m = plan.next(sh.rr, sh.k - 1, max_iter - launched);
+        } else {
+            m = std::min(cap, max_iter - launched);
         }
     }
     hipLaunchKernelGGL(k_cg_finish_x, dim3(vec_grid(c, n)), dim3(kVecThreads), 0, c->stream, c->p, x, n, c->state);

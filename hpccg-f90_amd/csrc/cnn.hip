@@ -1107,6 +1107,28 @@ int hpccg_fcg_learned_dev(hpccg_ctx *c, int32_t nx, int32_t ny, int32_t nz, cons
         hipLaunchKernelGGL(kf_update_p<false>, dim3(Gv), dim3(kVecThreads), 0, c->stream, F->z, F->p, n, F->st);
     };
 
+    // graphs of 1, 2, 4, ... iterations, launched in chunks that never cross a stagnation
+    // check point (every `window` iterations) and are sized from the residual rate
+    auto get = [&](int size, hipGraphExec_t *out) -> int {
+        GraphKey key{b, x, 200, size};
+        auto it = F->graphs.find(key);
+        if (it != F->graphs.end()) {
+            *out = it->second;
+            return 0;
+        }
+        hipGraph_t gr;
+        HPCCG_HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        for (int i = 0; i < size; ++i) iteration();
+        const hipError_t e = hipStreamEndCapture(c->stream, &gr);
+        HPCCG_HIP_TRY(e);
+        const hipError_t e2 = hipGraphInstantiate(out, gr, nullptr, nullptr, 0);
+        hipGraphDestroy(gr);
+        HPCCG_HIP_TRY(e2);
+        F->graphs[key] = *out;
+        return 0;
+    };
+    HPCCG_TRY(capture_iteration_graphs(window, max_iter, get));
+
     HPCCG_HIP_TRY(hipEventRecord(c->ev0, c->stream));
     // Prepare: r = b - A x, z = M(r), rz = z.r, p = z
     hipLaunchKernelGGL(kf_true<true>, dim3(Gt), dim3(kTrueThreads), 0, c->stream, sell16(A), x, b, F->r, F->st,
@@ -1120,33 +1142,19 @@ int hpccg_fcg_learned_dev(hpccg_ctx *c, int32_t nx, int32_t ny, int32_t nz, cons
     hipLaunchKernelGGL(kf_update_p<true>, dim3(Gv), dim3(kVecThreads), 0, c->stream, F->z, F->p, n, F->st);
     HPCCG_HIP_TRY(hipGetLastError());
 
-    GraphKey key{b, x, 200 + window, window};
-    hipGraphExec_t ge = nullptr;
-    auto it = F->graphs.find(key);
-    if (it != F->graphs.end()) {
-        ge = it->second;
-    } else if (max_iter > 0) {
-        hipGraph_t gr;
-        HPCCG_HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        for (int i = 0; i < window; ++i) iteration();
-        const hipError_t e = hipStreamEndCapture(c->stream, &gr);
-        HPCCG_HIP_TRY(e);
-        const hipError_t e2 = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
-        hipGraphDestroy(gr);
-        HPCCG_HIP_TRY(e2);
-        F->graphs[key] = ge;
-    }
-    double prev_rr = -1.0;
     bool fallback = false;
     FcgState &S = *F->st_h;
     HPCCG_HIP_TRY(hipMemcpyAsync(F->st_h, F->st, sizeof(FcgState), hipMemcpyDeviceToHost, c->stream));
     HPCCG_HIP_TRY(hipStreamSynchronize(c->stream));
-    prev_rr = S.rr;
+    double check_rr = S.rr;  // r.r at the last stagnation check point
+    int since = 0;           // iterations run since that point
     if (S.breakdown) fallback = !S.converged;
-    int launched = 0;
+    ChunkPlanner plan{window, tolerance, 0};
+    int m = std::min(window, max_iter), launched = 0;
     while (!fallback && !S.done && max_iter > 0) {
-        HPCCG_HIP_TRY(hipGraphLaunch(ge, c->stream));
-        launched += window;
+        const int k_before = S.k;
+        HPCCG_TRY(launch_iterations(c->stream, m, get));
+        launched += m;
         HPCCG_HIP_TRY(hipMemcpyAsync(F->st_h, F->st, sizeof(FcgState), hipMemcpyDeviceToHost, c->stream));
         HPCCG_HIP_TRY(hipStreamSynchronize(c->stream));
         if (S.breakdown) {
@@ -1154,13 +1162,18 @@ int hpccg_fcg_learned_dev(hpccg_ctx *c, int32_t nx, int32_t ny, int32_t nz, cons
             break;
         }
         if (S.done) break;
-        // stagnation: ||r|| shrank by less than min_factor over the last `window` iterations
-        if (!(std::sqrt(S.rr) * min_factor <= std::sqrt(prev_rr))) {
-            fallback = true;
-            break;
+        since += S.k - k_before;
+        if (since >= window) {
+            // stagnation: ||r|| shrank by less than min_factor over the last `window` iterations
+            if (!(std::sqrt(S.rr) * min_factor <= std::sqrt(check_rr))) {
+                fallback = true;
+                break;
+            }
+            check_rr = S.rr;
+            since = 0;
         }
-        prev_rr = S.rr;
         if (launched > 4 * max_iter + 64) break;
+        m = plan.next(S.rr, S.k - 1, std::min(window - since, std::max(1, max_iter - (S.k - 1))));
     }
     const int fcg_done = S.k - 1 < 0 ? 0 : (S.breakdown ? S.stop_at - 1 : S.k - 1);
     int cg_n = 0;

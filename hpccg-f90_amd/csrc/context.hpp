@@ -3,6 +3,8 @@
 
 #include <hip/hip_fp16.h>
 
+#include <algorithm>
+#include <cmath>
 #include <functional>
 #include <map>
 #include <tuple>
@@ -24,7 +26,61 @@ struct CgState {
     int breakdown;     // p.Ap was zero or non-finite
     unsigned counter[4];
     int pad[4];
+    double rr;         // ||r_{k-1}||^2, the last recorded r.r
 };
+
+// Sizes the graph launches of an iterative solve.  Each iteration skips itself on the device
+// once the solve has stopped, but a skipped iteration still costs its launches (~0.1 ms at
+// 256^3); so after every launch the host reads ||r|| and enqueues only the iterations it
+// predicts are left, from the geometric rate of the last launch, plus `lag` (iterations that
+// run after ||r|| < tol).  A short prediction costs one more small launch.
+struct ChunkPlanner {
+    int cap;     // most iterations per launch
+    double tol;  // absolute tolerance on ||r||; <= 0 or NaN: run to the iteration limit
+    int lag;
+    double rr0 = -1.0;
+    int k0 = 0;
+    double rate = 0.0;  // per-iteration factor of ||r||
+    // rr = r.r after iteration k; left = iterations before the limit
+    int next(double rr, int k, int left)
+    {
+        if (rr0 > 0.0 && rr > 0.0 && k > k0) rate = std::pow(rr / rr0, 0.5 / (double)(k - k0));
+        rr0 = rr;
+        k0 = k;
+        int m = std::min(cap, left);
+        if (tol > 0.0 && rate > 0.0 && rate < 1.0 && rr > 0.0) {
+            const double need = std::log(tol / std::sqrt(rr)) / std::log(rate);
+            if (need < (double)m) m = std::max(0, (int)std::ceil(need)) + lag;
+        }
+        return std::max(1, std::min(m, std::min(cap, left)));
+    }
+};
+
+// Captures the graphs of 1, 2, 4, ... <= min(cap, max_iter) iterations (get(size, &exec)
+// returns the cached graph of `size` iterations, capturing it on first use); called before
+// the timed region and before anything is queued on the stream.
+template <typename Get>
+int capture_iteration_graphs(int cap, int max_iter, Get get)
+{
+    for (int size = 1; size <= cap && size <= max_iter; size *= 2) {
+        hipGraphExec_t ge = nullptr;
+        HPCCG_TRY(get(size, &ge));
+    }
+    return 0;
+}
+
+// Enqueues m iterations as graphs of power-of-two sizes, all captured beforehand.
+template <typename Get>
+int launch_iterations(hipStream_t s, int m, Get get)
+{
+    for (int bit = 30; bit >= 0; --bit) {
+        if (!(m & (1 << bit))) continue;
+        hipGraphExec_t ge = nullptr;
+        HPCCG_TRY(get(1 << bit, &ge));
+        HPCCG_HIP_TRY(hipGraphLaunch(ge, s));
+    }
+    return 0;
+}
 
 struct Matrix {
     int32_t nrow = 0, ncol = 0;

@@ -379,18 +379,19 @@ static int cg_mixed_device(hpccg_ctx *c, const double *b, double *x, int max_ite
     std::vector<double> neg(2 * (size_t)m.hist_cap, -1.0);
     HPCCG_HIP_TRY(hipMemcpyAsync(m.hist, neg.data(), neg.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
 
-    hipGraphExec_t ge = nullptr;
-    const int chunk = std::max(1, std::min(max_iter, 16));
-    if (max_iter > 0) HPCCG_TRY(mixed_graph(c, b, x, chunk, &ge));
+    // graphs of up to 16 iterations, sized by the host from the residual rate (ChunkPlanner)
+    auto get = [&](int size, hipGraphExec_t *ge) { return mixed_graph(c, b, x, size, ge); };
+    HPCCG_TRY(capture_iteration_graphs(16, max_iter, get));
+    ChunkPlanner plan{16, tol, 0};
 
     HPCCG_HIP_TRY(hipEventRecord(c->ev0, c->stream));
     launch_km_true<true>(c, b, x);
     HPCCG_HIP_TRY(hipGetLastError());
-    int launched = 0;
+    int launched = 0, mm = std::min(16, max_iter);
     MixState *sh = (MixState *)m.state_h;
     while (max_iter > 0) {
-        HPCCG_HIP_TRY(hipGraphLaunch(ge, c->stream));
-        launched += chunk;
+        HPCCG_TRY(launch_iterations(c->stream, mm, get));
+        launched += mm;
         HPCCG_HIP_TRY(hipMemcpyAsync(m.state_h, m.state, sizeof(MixState), hipMemcpyDeviceToHost, c->stream));
         HPCCG_HIP_TRY(hipStreamSynchronize(c->stream));
         if (sh->k >= sh->stop_at) break;
@@ -398,6 +399,8 @@ static int cg_mixed_device(hpccg_ctx *c, const double *b, double *x, int max_ite
             set_error("mixed cg: iteration counter did not advance");
             return HPCCG_ERR_STATE;
         }
+        // iterations whose true residual is still due stay in the count (k does not advance)
+        mm = plan.next(sh->rtrans, sh->k - 1, std::max(1, max_iter - (sh->k - 1)));
     }
     HPCCG_HIP_TRY(hipEventRecord(c->ev1, c->stream));
     HPCCG_HIP_TRY(hipMemcpyAsync(m.state_h, m.state, sizeof(MixState), hipMemcpyDeviceToHost, c->stream));
