@@ -128,8 +128,9 @@ int hpccg_cg_dev(hpccg_ctx *ctx, const double *b, double *x, int32_t max_iter, d
  * FP16 matrix values (exact for generate_matrix's 27 / -1; other matrices are rejected unless
  * every value is an FP16 number), int32 columns, FP32 r/p/Ap with FP64 arithmetic inside the
  * kernels, FP64 dots for alpha/beta, FP64 x, and replacement of the recursive residual by the
- * FP64 true residual b - A x every replace_every iterations (0 = never; the true residual is
- * still formed whenever the recursive norm falls below the tolerance and at max_iter).
+ * FP64 true residual b - A x every replace_every iterations (0 = no periodic replacement; the
+ * true residual is still formed whenever the recursive norm falls below the tolerance, at
+ * max_iter, and once the recursive r.r has fallen 1e10-fold since the last true one).
  * Stops when the TRUE residual norm is below `tolerance` (absolute).  niters = iterations run;
  * *true_normr = ||b - A x|| at exit; hist_rec[j] (j = 1..max_iter) the recursive ||r_j||;
  * hist_true[j] (j = 0..max_iter) the true ||b - A x_j|| where formed, else -1. */
@@ -207,7 +208,8 @@ int hpccg_cnn_train(hpccg_ctx *ctx, int32_t steps, int32_t batch, float lr, uint
 int hpccg_cnn_apply_dev(hpccg_ctx *ctx, int32_t nx, int32_t ny, int32_t nz, const float *r,
                         float inv_scale, float *z);
 /* Flexible CG preconditioned by the network (FP16 matrix values, FP32 r/p/Ap/z, FP64 dots and
- * x, true-residual replacement every replace_every iterations, stop on the FP64 true residual
+ * x, true-residual replacement every replace_every iterations and after every 1e10-fold fall
+ * of the recursive r.r, as in hpccg_cg_mixed; stop on the FP64 true residual
  * ||b - A x|| <= tolerance).  Every `window` iterations ||r|| must have dropped by min_factor,
  * else (or on a breakdown) the current x is handed to FP64 CG (hpccg_cg_dev semantics) for the
  * rest of the solve.  (nx,ny,nz) is the matrix's grid.  hist_rec[0..max_iter] (may be NULL)
