@@ -271,6 +271,28 @@ __device__ __forceinline__ void elem_range(int64_t n, int64_t &i0, int64_t &i1, 
     i1 = (n * (lb + 1)) / G;
 }
 
+// Element loops of the FP32 vector kernels: the workgroup's range starts on a multiple of 4
+// (the last logical workgroup takes the remainder) so that its body reads and writes whole
+// 16-byte float4s; each thread takes every kVecThreads-th group of four, then a scalar tail.
+__device__ __forceinline__ void elem_range4(int64_t n, int64_t &i0, int64_t &i1, int &lb)
+{
+    const int G = gridDim.x;
+    lb = xcd_logical_block(blockIdx.x, G);
+    const int64_t q = n / 4;
+    i0 = 4 * ((q * lb) / G);
+    i1 = lb + 1 == G ? n : 4 * ((q * (lb + 1)) / G);
+}
+template <typename F4, typename F1>
+__device__ __forceinline__ void vec4_loop(int64_t i0, int64_t i1, F4 body4, F1 body1)
+{
+    const int64_t iv = i0 + ((i1 - i0) & ~(int64_t)3);
+    for (int64_t i = i0 + 4 * (int64_t)threadIdx.x; i < iv; i += 4 * (int64_t)blockDim.x) body4(i);
+    for (int64_t i = iv + threadIdx.x; i < i1; i += blockDim.x) body1(i);
+}
+typedef float f32x4v __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ f32x4v ld4(const float *p) { return *(const f32x4v *)p; }
+__device__ __forceinline__ void st4(float *p, f32x4v v) { *(f32x4v *)p = v; }
+
 // Launch shapes (host).  SpMV kernels launch one wave per slice, kSellWaves slices per
 // workgroup, in slice order (the hardware dispatches workgroups in order, so the waves in
 // flight stream a compact window of the matrix); the vector kernels use a resident grid.
