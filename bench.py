@@ -236,17 +236,17 @@ def main():
         t_tune = time.perf_counter()
         best, var_ms = ctx.batch_autotune(bb, xbd, probe_iters=8)
         t_tune = time.perf_counter() - t_tune
-        ctx.batch_solve_dev(bb, xbd, rel_tol=1e-8, max_iter=400)  # capture + warm
+        ctx.batch_solve_dev(bb, xbd, rel_tol=1e-8, max_iter=200)  # capture + warm
         xbd.upload(np.zeros(binfo["rows"]))
         t = time.perf_counter()
-        res = ctx.batch_solve_dev(bb, xbd, rel_tol=1e-8, max_iter=400)
+        res = ctx.batch_solve_dev(bb, xbd, rel_tol=1e-8, max_iter=200)
         t = time.perf_counter() - t
         variants = ctx.batch_variants()
         batched = {"systems": 64, "rows": binfo["rows"], "nnz": binfo["nnz"], "s_wall": round(t, 6),
                    "s_device": round(res["ms"] / 1e3, 6), "converged": int(res["converged"].sum()),
                    "max_relres": float(res["relres"].max()), "iters_min_max": [int(res["niters"].min()),
                                                                                int(res["niters"].max())],
-                   "autotune_s": round(t_tune, 3), "variant": {"rows": variants[best][0], "wg_per_cu": variants[best][1]},
+                   "autotune_s": round(t_tune, 3), "variant": {"rows_per_wave": variants[best][0], "waves_per_wg": variants[best][1]},
                    "variant_ms": [round(v, 4) for v in var_ms.tolist()]}
         log(f"batched: {batched}")
 

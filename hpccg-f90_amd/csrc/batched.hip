@@ -5,14 +5,16 @@
 // mixed.hip (FP16 values, FP32 r/p/Ap, FP64 dots and x, FP64 true-residual replacement).
 //
 // Layout: the systems' CSRs are concatenated into one block-diagonal CSR (global 1-based
-// row pointers and columns), vectors are concatenated by row offset.  Every kernel covers all
-// systems at once: system s owns a "team" of g_s workgroups (g_s proportional to its row
-// tiles), each team reduces its own dots with the single-pass last-block pattern, and a
-// system that has converged makes its team's workgroups return at once.  The whole solve --
-// Prepare and max_iter iterations -- is captured as ONE graph and replayed with one launch.
+// row pointers and columns), vectors are concatenated by row offset, and the sliced-ELL copy
+// starts every system on a new 64-row slice.  Every kernel covers all systems at once and is
+// slice-parallel: a wave owns `rows per wave` / 64 consecutive slices of the whole batch and
+// writes one dot partial per slice; a per-system finish kernel (one workgroup per system) sums
+// its slices' partials in a fixed order and updates that system's scalars.  A converged
+// system's slices return at once.  The whole solve -- Prepare and max_iter iterations -- is
+// captured as ONE graph and replayed with one launch.
 //
-// Launch shape (rows per workgroup tile = rows per wave x waves, and workgroups per CU) is
-// chosen per batch by hpccg_batch_autotune, which times every compiled variant on the batch.
+// Launch shape (rows per wave and waves per workgroup) is chosen per batch by
+// hpccg_batch_autotune, which times every compiled variant on the batch.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -39,14 +41,6 @@ struct BatchState {
     unsigned counter[4];
 };
 
-struct BatchLayout {  // device arrays describing systems and teams for one variant
-    int *wg_team = nullptr;     // [G] team (system) of each logical workgroup, -1 = idle
-    int *team_first = nullptr;  // [S] first logical workgroup of each team
-    int *team_size = nullptr;   // [S]
-    int G = 0;
-    int rows = 0, occ = 0;
-};
-
 struct Batch {
     int S = 0;
     std::vector<int32_t> dims;          // 3*S
@@ -61,12 +55,12 @@ struct Batch {
     int32_t *scol = nullptr;            // 0-based global columns
     __half *sval = nullptr;
     int *slice_off = nullptr;           // [S+1] first slice of each system
+    int *slice_sys = nullptr;           // [NS] system of each slice
+    int NS = 0;                         // slices of the whole batch
     int64_t slots = 0;
     float *r = nullptr, *p = nullptr, *Ap = nullptr;
     BatchState *st = nullptr;
-    double *partials = nullptr;         // one slot per logical workgroup
-    int partials_cap = 0;
-    BatchLayout lay[8];
+    double *partials = nullptr;         // [2 NS]: per-slice dot partials
     int best_variant = -1;
     hipGraphExec_t graph = nullptr;
     const void *g_b = nullptr, *g_x = nullptr;
@@ -74,11 +68,11 @@ struct Batch {
     double g_tol2 = -1.0;
 };
 
-// compiled variants: rows per tile (one row per lane) x workgroups per CU
+// compiled variants: waves per workgroup x slices per wave (rows per wave = 64 x slices)
 struct Variant {
-    int rows, occ;
+    int waves, spw;
 };
-static const Variant kVariants[] = {{64, 8}, {64, 16}, {128, 4}, {128, 8}, {256, 2}, {256, 4}};
+static const Variant kVariants[] = {{4, 1}, {8, 1}, {2, 1}, {16, 1}, {4, 2}, {8, 2}, {4, 4}, {8, 4}};
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
 static Batch *batch_of(hpccg_ctx *c) { return (Batch *)c->ext[0]; }
@@ -115,212 +109,227 @@ __global__ void kb_generate(int nx, int ny, int nz, int64_t row_off, int64_t nnz
 
 __global__ void kb_set_last(int32_t *rowindex, int64_t E, int64_t nnz) { rowindex[E] = (int32_t)(nnz + 1); }
 
-// ---- per-team helpers -------------------------------------------------------------------------
-struct TeamPos {
-    int s, l, g;
-    int64_t off, n;
-};
-
-__device__ __forceinline__ bool team_pos(const int *wg_team, const int *team_first, const int *team_size,
-                                         const int64_t *row_off, TeamPos &tp, int &lb)
-{
-    lb = xcd_logical_block(blockIdx.x, gridDim.x);
-    const int s = wg_team[lb];
-    if (s < 0) return false;
-    tp.s = s;
-    tp.l = lb - team_first[s];
-    tp.g = team_size[s];
-    tp.off = row_off[s];
-    tp.n = row_off[s + 1] - row_off[s];
-    return true;
-}
-
-// The batch's sliced copy as the SpMV kernels take it.
+// ---- slice helpers -----------------------------------------------------------------------------
+// The batch's sliced copy as the kernels take it.
 struct BatchSell {
     const int64_t *slice_ptr;
     const int32_t *col;
     const __half *val;
     const int32_t *rowptr;  // global 1-based row pointer
     const int *slice_off;   // first slice of each system
+    const int *slice_sys;   // system of each slice
+    const int64_t *row_off; // first row of each system
+    int NS;
 };
 
-static inline BatchSell batch_sell(const Batch *B) { return BatchSell{B->slice_ptr, B->scol, B->sval, B->rowindex, B->slice_off}; }
-
-// A team's workgroup walks tiles of ROWS/64 slices of its system (one slice per wave) over a
-// contiguous tile range; f(global row, row sum) runs for every row of the system it covers.
-template <int ROWS, typename XT, typename F>
-__device__ __forceinline__ void team_rows(const BatchSell &A, const TeamPos &tp, const XT *__restrict__ x, F f)
+static inline BatchSell batch_sell(const Batch *B)
 {
-    constexpr int W = ROWS / kSlice;
-    const int nsl = (int)((tp.n + kSlice - 1) / kSlice);
-    const int ntiles = (nsl + W - 1) / W;
-    const int t0 = (int)(((int64_t)tp.l * ntiles) / tp.g), t1 = (int)(((int64_t)(tp.l + 1) * ntiles) / tp.g);
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kSlice), lane = threadIdx.x % kSlice;
-    const int soff = A.slice_off[tp.s];
-    for (int t = t0; t < t1; ++t) {
-        const int ls = t * W + w;
-        if (ls >= nsl) continue;  // wave-uniform
-        const int64_t rl = (int64_t)ls * kSlice + lane;
-        const bool valid = rl < tp.n;
-        const int64_t gr = tp.off + rl;
-        const double sum = sell_rowsum<kGroupF16, __half, XT>(A.slice_ptr, A.col, A.val, A.rowptr, x, soff + ls, lane,
-                                                             gr, valid);
-        if (valid) f(gr, sum);
-    }
+    return BatchSell{B->slice_ptr, B->scol, B->sval, B->rowindex, B->slice_off, B->slice_sys, B->row_off, B->NS};
+}
+
+// Where slice gs lies: its system, the lane's global row and whether that row exists.
+struct SlicePos {
+    int s;
+    int64_t gr;
+    bool valid;
+};
+__device__ __forceinline__ SlicePos slice_pos(const BatchSell &A, int gs, int lane)
+{
+    SlicePos q;
+    q.s = A.slice_sys[gs];
+    const int64_t off = A.row_off[q.s], n = A.row_off[q.s + 1] - off;
+    const int64_t rl = (int64_t)(gs - A.slice_off[q.s]) * kSlice + lane;
+    q.valid = rl < n;
+    q.gr = off + rl;
+    return q;
+}
+
+// The SPW consecutive slices of this wave: f(gs) for each that exists.
+template <int W, int SPW, typename F>
+__device__ __forceinline__ void wave_slices(int NS, F f)
+{
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kSlice);
+    const int g0 = (blockIdx.x * W + w) * SPW;
+#pragma unroll
+    for (int j = 0; j < SPW; ++j)
+        if (g0 + j < NS) f(g0 + j);
+}
+
+// Sum of the slice partials [slice_off[s], slice_off[s+1]) (stride `stride`, offset `o`) in
+// a fixed order; every thread of the 256-thread workgroup gets it.
+__device__ __forceinline__ double system_sum(const double *__restrict__ part, const int *__restrict__ slice_off, int s,
+                                             int stride, int o, double *red)
+{
+    double t = 0.0;
+    for (int i = slice_off[s] + threadIdx.x; i < slice_off[s + 1]; i += 256) t += part[(int64_t)stride * i + o];
+    return block_sum<256>(t, red);
 }
 
 // ---- kernels ------------------------------------------------------------------------------------
-// True residual of every system that asked for one (PREP: all systems, also forms ||b||^2).
-template <int ROWS, bool PREP>
-__global__ __launch_bounds__(ROWS) void kb_true(const BatchSell A, const double *__restrict__ x,
-                                                const double *__restrict__ b, float *__restrict__ r,
-                                                const int64_t *__restrict__ row_off, const int *wg_team,
-                                                const int *team_first, const int *team_size, BatchState *st,
-                                                double *partials, double rel_tol2)
+// True residual r = b - A x of every system that asked for one (PREP: all systems, also
+// ||b||^2), per-slice partials (r.r, b.b) at part[2 gs], part[2 gs + 1].
+template <int W, int SPW, bool PREP>
+__global__ __launch_bounds__(W * 64) void kb_true(const BatchSell A, const double *__restrict__ x,
+                                                   const double *__restrict__ b, float *__restrict__ r,
+                                                   const BatchState *st, double *part)
 {
-    __shared__ double red[ROWS / kWave];
-    TeamPos tp;
-    int lb;
-    if (!team_pos(wg_team, team_first, team_size, row_off, tp, lb)) return;
-    BatchState *S = st + tp.s;
-    if (!PREP && (!S->active || !S->need_true)) return;
-    double rr = 0.0, bb = 0.0;
-    team_rows<ROWS>(A, tp, x, [&](int64_t gr, double ax) {
-        const double bi = b[gr];
-        const float ri = (float)(bi - ax);
-        __builtin_nontemporal_store(ri, r + gr);
-        rr = fma((double)ri, (double)ri, rr);
-        if (PREP) bb = fma(bi, bi, bb);
-    });
-    rr = block_sum<ROWS>(rr, red);
-    if (PREP) bb = block_sum<ROWS>(bb, red);
-    __shared__ int is_last;
-    if (threadIdx.x == 0) {
-        partials[2 * lb] = rr;
-        partials[2 * lb + 1] = bb;
-        __threadfence();
-        is_last = atomicAdd(&S->counter[0], 1u) == (unsigned)tp.g - 1u;
-    }
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();
-    double a = 0.0, c2 = 0.0;
-    const int first = lb - tp.l;
-    for (int i = threadIdx.x; i < tp.g; i += ROWS) {
-        a += load_agent(partials + 2 * (first + i));
-        c2 += load_agent(partials + 2 * (first + i) + 1);
-    }
-    a = block_sum<ROWS>(a, red);
-    c2 = block_sum<ROWS>(c2, red);
-    if (threadIdx.x == 0) {
-        __hip_atomic_store(&S->counter[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        S->rr_true = a;
-        if (PREP) {
-            S->bb = c2;
-            S->tol2 = rel_tol2 * c2;
-            S->rtrans = a;
-            S->k = 1;
-            S->niters = 0;
-            S->active = (a < S->tol2) ? 0 : (S->max_iter > 0 ? 1 : 0);
-            S->converged = a < S->tol2;
-        } else {
-            const int k = S->k;
-            S->niters = k;
-            if (a < S->tol2) {
-                S->converged = 1;
-                S->active = 0;
-            } else if (k >= S->max_iter) {
-                S->active = 0;
-            }
-            S->beta = a / S->rtrans;
-            S->rtrans = a;
-            S->need_true = 0;
-            S->k = k + 1;
+    const int lane = threadIdx.x % kSlice;
+    wave_slices<W, SPW>(A.NS, [&](int gs) {
+        const SlicePos q = slice_pos(A, gs, lane);
+        const BatchState *S = st + q.s;
+        if (!PREP && (!S->active || !S->need_true)) return;
+        const double ax =
+            sell_rowsum<kGroupF16, __half, double>(A.slice_ptr, A.col, A.val, A.rowptr, x, gs, lane, q.gr, q.valid);
+        double rr = 0.0, bb = 0.0;
+        if (q.valid) {
+            const double bi = b[q.gr];
+            const float ri = (float)(bi - ax);
+            __builtin_nontemporal_store(ri, r + q.gr);
+            rr = (double)ri * (double)ri;
+            bb = bi * bi;
         }
-    }
-}
-
-template <int ROWS>
-__global__ __launch_bounds__(ROWS) void kb_update_p(const float *__restrict__ r, float *__restrict__ p,
-                                                    const int64_t *__restrict__ row_off, const int *wg_team,
-                                                    const int *team_first, const int *team_size, const BatchState *st)
-{
-    TeamPos tp;
-    int lb;
-    if (!team_pos(wg_team, team_first, team_size, row_off, tp, lb)) return;
-    const BatchState *S = st + tp.s;
-    if (!S->active) return;
-    const int64_t i0 = tp.off + (tp.n * tp.l) / tp.g, i1 = tp.off + (tp.n * (tp.l + 1)) / tp.g;
-    if (S->k == 1) {
-        for (int64_t i = i0 + threadIdx.x; i < i1; i += ROWS) p[i] = r[i];
-        return;
-    }
-    const double beta = S->beta;
-    for (int64_t i = i0 + threadIdx.x; i < i1; i += ROWS) p[i] = (float)fma(beta, (double)p[i], (double)r[i]);
-}
-
-template <int ROWS>
-__global__ __launch_bounds__(ROWS) void kb_spmv(const BatchSell A, const float *__restrict__ p, float *__restrict__ Ap,
-                                                const int64_t *__restrict__ row_off, const int *wg_team,
-                                                const int *team_first, const int *team_size, BatchState *st,
-                                                double *partials)
-{
-    __shared__ double red[ROWS / kWave];
-    TeamPos tp;
-    int lb;
-    if (!team_pos(wg_team, team_first, team_size, row_off, tp, lb)) return;
-    BatchState *S = st + tp.s;
-    if (!S->active) return;
-    double dot = 0.0;
-    team_rows<ROWS>(A, tp, p, [&](int64_t gr, double s) {
-        const float a = (float)s;
-        __builtin_nontemporal_store(a, Ap + gr);
-        dot = fma((double)p[gr], (double)a, dot);
+        rr = wave_sum(rr);
+        if (PREP) bb = wave_sum(bb);
+        if (lane == 0) {
+            part[2 * (int64_t)gs] = rr;
+            part[2 * (int64_t)gs + 1] = bb;
+        }
     });
-    dot = block_sum<ROWS>(dot, red);
-    double pap;
-    if (grid_sum<ROWS>(dot, partials + (lb - tp.l), &S->counter[1], tp.g, tp.l, red, &pap) && threadIdx.x == 0) {
+}
+
+// one workgroup per system: the true residual's norm, convergence, the iteration count
+template <bool PREP>
+__global__ __launch_bounds__(256) void kb_true_finish(const double *__restrict__ part, const int *__restrict__ slice_off,
+                                                      BatchState *st, double rel_tol2)
+{
+    __shared__ double red[4];
+    const int s = blockIdx.x;
+    BatchState *S = st + s;
+    if (!PREP && (!S->active || !S->need_true)) return;
+    const double a = system_sum(part, slice_off, s, 2, 0, red);
+    const double c2 = PREP ? system_sum(part, slice_off, s, 2, 1, red) : 0.0;
+    if (threadIdx.x != 0) return;
+    S->rr_true = a;
+    if (PREP) {
+        S->bb = c2;
+        S->tol2 = rel_tol2 * c2;
+        S->rtrans = a;
+        S->k = 1;
+        S->niters = 0;
+        S->active = (a < S->tol2) ? 0 : (S->max_iter > 0 ? 1 : 0);
+        S->converged = a < S->tol2;
+    } else {
+        const int k = S->k;
+        S->niters = k;
+        if (a < S->tol2) {
+            S->converged = 1;
+            S->active = 0;
+        } else if (k >= S->max_iter) {
+            S->active = 0;
+        }
+        S->beta = a / S->rtrans;
+        S->rtrans = a;
+        S->need_true = 0;
+        S->k = k + 1;
+    }
+}
+
+// Top of iteration k: p = r (k == 1) or p = r + beta p.
+template <int W, int SPW>
+__global__ __launch_bounds__(W * 64) void kb_update_p(const BatchSell A, const float *__restrict__ r,
+                                                       float *__restrict__ p, const BatchState *st)
+{
+    const int lane = threadIdx.x % kSlice;
+    wave_slices<W, SPW>(A.NS, [&](int gs) {
+        const SlicePos q = slice_pos(A, gs, lane);
+        const BatchState *S = st + q.s;
+        if (!S->active || !q.valid) return;
+        const float ri = r[q.gr];
+        p[q.gr] = S->k == 1 ? ri : (float)fma(S->beta, (double)p[q.gr], (double)ri);
+    });
+}
+
+// Ap = A p and the slice's p.Ap partial at part[gs]
+template <int W, int SPW>
+__global__ __launch_bounds__(W * 64) void kb_spmv(const BatchSell A, const float *__restrict__ p,
+                                                   float *__restrict__ Ap, const BatchState *st, double *part)
+{
+    const int lane = threadIdx.x % kSlice;
+    wave_slices<W, SPW>(A.NS, [&](int gs) {
+        const SlicePos q = slice_pos(A, gs, lane);
+        if (!st[q.s].active) return;
+        const double sum =
+            sell_rowsum<kGroupF16, __half, float>(A.slice_ptr, A.col, A.val, A.rowptr, p, gs, lane, q.gr, q.valid);
+        double dot = 0.0;
+        if (q.valid) {
+            const float a = (float)sum;
+            __builtin_nontemporal_store(a, Ap + q.gr);
+            dot = (double)p[q.gr] * (double)a;
+        }
+        dot = wave_sum(dot);
+        if (lane == 0) part[gs] = dot;
+    });
+}
+
+// one workgroup per system: alpha = rtrans / p.Ap
+__global__ __launch_bounds__(256) void kb_pap_finish(const double *__restrict__ part, const int *__restrict__ slice_off,
+                                                     BatchState *st)
+{
+    __shared__ double red[4];
+    BatchState *S = st + blockIdx.x;
+    if (!S->active) return;
+    const double pap = system_sum(part, slice_off, blockIdx.x, 1, 0, red);
+    if (threadIdx.x == 0) {
         if (!(pap != 0.0) || !isfinite(pap)) S->breakdown = 1;
         S->alpha = S->rtrans / pap;
     }
 }
 
-template <int ROWS>
-__global__ __launch_bounds__(ROWS) void kb_update_xr(double *__restrict__ x, float *__restrict__ r,
-                                                     const float *__restrict__ p, const float *__restrict__ Ap,
-                                                     const int64_t *__restrict__ row_off, const int *wg_team,
-                                                     const int *team_first, const int *team_size, BatchState *st,
-                                                     double *partials)
+// x = x + alpha p (FP64), r = r - alpha Ap (FP32), the slice's r.r partial at part[gs]
+template <int W, int SPW>
+__global__ __launch_bounds__(W * 64) void kb_update_xr(const BatchSell A, double *__restrict__ x,
+                                                        float *__restrict__ r, const float *__restrict__ p,
+                                                        const float *__restrict__ Ap, const BatchState *st,
+                                                        double *part)
 {
-    __shared__ double red[ROWS / kWave];
-    TeamPos tp;
-    int lb;
-    if (!team_pos(wg_team, team_first, team_size, row_off, tp, lb)) return;
-    BatchState *S = st + tp.s;
-    if (!S->active) return;
-    const double alpha = S->alpha;
-    const int64_t i0 = tp.off + (tp.n * tp.l) / tp.g, i1 = tp.off + (tp.n * (tp.l + 1)) / tp.g;
-    double dot = 0.0;
-    for (int64_t i = i0 + threadIdx.x; i < i1; i += ROWS) {
-        x[i] = fma(alpha, (double)p[i], x[i]);
-        const float ri = (float)fma(-alpha, (double)Ap[i], (double)r[i]);
-        r[i] = ri;
-        dot = fma((double)ri, (double)ri, dot);
-    }
-    dot = block_sum<ROWS>(dot, red);
-    double rr;
-    if (grid_sum<ROWS>(dot, partials + (lb - tp.l), &S->counter[2], tp.g, tp.l, red, &rr) && threadIdx.x == 0) {
-        const int k = S->k;
-        const bool check = (S->replace_every > 0 && k % S->replace_every == 0) || rr < S->tol2 ||
-                           k >= S->max_iter || rr < kReplaceDrop2 * S->rr_true;
-        if (check) {
-            S->need_true = 1;
-        } else {
-            S->niters = k;
-            S->beta = rr / S->rtrans;
-            S->rtrans = rr;
-            S->k = k + 1;
+    const int lane = threadIdx.x % kSlice;
+    wave_slices<W, SPW>(A.NS, [&](int gs) {
+        const SlicePos q = slice_pos(A, gs, lane);
+        const BatchState *S = st + q.s;
+        if (!S->active) return;
+        double dot = 0.0;
+        if (q.valid) {
+            const double alpha = S->alpha;
+            x[q.gr] = fma(alpha, (double)p[q.gr], x[q.gr]);
+            const float ri = (float)fma(-alpha, (double)Ap[q.gr], (double)r[q.gr]);
+            r[q.gr] = ri;
+            dot = (double)ri * (double)ri;
         }
+        dot = wave_sum(dot);
+        if (lane == 0) part[gs] = dot;
+    });
+}
+
+// one workgroup per system: recursive r.r; a true residual next (every replace_every
+// iterations, after a kReplaceDrop2 fall, at the tolerance, at max_iter) or beta and k + 1
+__global__ __launch_bounds__(256) void kb_xr_finish(const double *__restrict__ part, const int *__restrict__ slice_off,
+                                                    BatchState *st)
+{
+    __shared__ double red[4];
+    BatchState *S = st + blockIdx.x;
+    if (!S->active) return;
+    const double rr = system_sum(part, slice_off, blockIdx.x, 1, 0, red);
+    if (threadIdx.x != 0) return;
+    const int k = S->k;
+    const bool check = (S->replace_every > 0 && k % S->replace_every == 0) || rr < S->tol2 || k >= S->max_iter ||
+                       rr < kReplaceDrop2 * S->rr_true;
+    if (check) {
+        S->need_true = 1;
+    } else {
+        S->niters = k;
+        S->beta = rr / S->rtrans;
+        S->rtrans = rr;
+        S->k = k + 1;
     }
 }
 
@@ -343,87 +352,41 @@ void free_batched(hpccg_ctx *c)
     hipFree(B->Ap);
     hipFree(B->st);
     hipFree(B->partials);
-    for (auto &L : B->lay) {
-        hipFree(L.wg_team);
-        hipFree(L.team_first);
-        hipFree(L.team_size);
-    }
+    hipFree(B->slice_sys);
     delete B;
     c->ext[0] = nullptr;
 }
 
-static int build_layout(hpccg_ctx *c, Batch *B, int v)
+template <int W, int SPW>
+static void enqueue_batch(hpccg_ctx *c, Batch *B, const double *b, double *x, int max_iter, double rel_tol2)
 {
-    BatchLayout &L = B->lay[v];
-    if (L.wg_team) return 0;
-    const int rows = kVariants[v].rows, occ = kVariants[v].occ;
-    std::vector<int64_t> tiles(B->S);
-    int64_t T = 0;
-    for (int s = 0; s < B->S; ++s) {
-        tiles[s] = (B->row_off_h[s + 1] - B->row_off_h[s] + rows - 1) / rows;
-        T += tiles[s];
-    }
-    const double target = (double)c->info.num_cu * occ;
-    std::vector<int> first(B->S), size(B->S);
-    int G = 0;
-    for (int s = 0; s < B->S; ++s) {
-        int g = (int)std::lround(target * (double)tiles[s] / (double)T);
-        g = std::max(1, std::min<int>(g, (int)tiles[s]));
-        first[s] = G;
-        size[s] = g;
-        G += g;
-    }
-    const int Gp = (G + kNumXcd - 1) / kNumXcd * kNumXcd;
-    std::vector<int> team(Gp, -1);
-    for (int s = 0; s < B->S; ++s)
-        for (int i = 0; i < size[s]; ++i) team[first[s] + i] = s;
-    HPCCG_HIP_TRY(hipMalloc(&L.wg_team, Gp * sizeof(int)));
-    HPCCG_HIP_TRY(hipMalloc(&L.team_first, B->S * sizeof(int)));
-    HPCCG_HIP_TRY(hipMalloc(&L.team_size, B->S * sizeof(int)));
-    HPCCG_HIP_TRY(hipMemcpy(L.wg_team, team.data(), Gp * sizeof(int), hipMemcpyHostToDevice));
-    HPCCG_HIP_TRY(hipMemcpy(L.team_first, first.data(), B->S * sizeof(int), hipMemcpyHostToDevice));
-    HPCCG_HIP_TRY(hipMemcpy(L.team_size, size.data(), B->S * sizeof(int), hipMemcpyHostToDevice));
-    L.G = Gp;
-    L.rows = rows;
-    L.occ = occ;
-    if (2 * Gp > B->partials_cap) {
-        hipFree(B->partials);
-        B->partials = nullptr;
-        HPCCG_HIP_TRY(hipMalloc(&B->partials, 2 * (size_t)Gp * sizeof(double)));
-        B->partials_cap = 2 * Gp;
-        if (B->graph) hipGraphExecDestroy(B->graph);
-        B->graph = nullptr;
-    }
-    return 0;
-}
-
-template <int ROWS>
-static void enqueue_batch(hpccg_ctx *c, Batch *B, const BatchLayout &L, const double *b, double *x, int max_iter,
-                          double rel_tol2)
-{
-    const dim3 grid(L.G), blk(ROWS);
+    const dim3 grid((unsigned)((B->NS + W * SPW - 1) / (W * SPW))), blk(W * 64), sys((unsigned)B->S), fin(256);
     const BatchSell A = batch_sell(B);
-    hipLaunchKernelGGL((kb_true<ROWS, true>), grid, blk, 0, c->stream, A, x, b, B->r, B->row_off, L.wg_team,
-                       L.team_first, L.team_size, B->st, B->partials, rel_tol2);
+    double *part = B->partials;
+    hipLaunchKernelGGL((kb_true<W, SPW, true>), grid, blk, 0, c->stream, A, x, b, B->r, B->st, part);
+    hipLaunchKernelGGL(kb_true_finish<true>, sys, fin, 0, c->stream, part, B->slice_off, B->st, rel_tol2);
     for (int it = 0; it < max_iter; ++it) {
-        hipLaunchKernelGGL(kb_update_p<ROWS>, grid, blk, 0, c->stream, B->r, B->p, B->row_off, L.wg_team,
-                           L.team_first, L.team_size, B->st);
-        hipLaunchKernelGGL(kb_spmv<ROWS>, grid, blk, 0, c->stream, A, B->p, B->Ap, B->row_off, L.wg_team,
-                           L.team_first, L.team_size, B->st, B->partials);
-        hipLaunchKernelGGL(kb_update_xr<ROWS>, grid, blk, 0, c->stream, x, B->r, B->p, B->Ap, B->row_off, L.wg_team,
-                           L.team_first, L.team_size, B->st, B->partials);
-        hipLaunchKernelGGL((kb_true<ROWS, false>), grid, blk, 0, c->stream, A, x, b, B->r, B->row_off, L.wg_team,
-                           L.team_first, L.team_size, B->st, B->partials, rel_tol2);
+        hipLaunchKernelGGL((kb_update_p<W, SPW>), grid, blk, 0, c->stream, A, B->r, B->p, B->st);
+        hipLaunchKernelGGL((kb_spmv<W, SPW>), grid, blk, 0, c->stream, A, B->p, B->Ap, B->st, part);
+        hipLaunchKernelGGL(kb_pap_finish, sys, fin, 0, c->stream, part, B->slice_off, B->st);
+        hipLaunchKernelGGL((kb_update_xr<W, SPW>), grid, blk, 0, c->stream, A, x, B->r, B->p, B->Ap, B->st, part);
+        hipLaunchKernelGGL(kb_xr_finish, sys, fin, 0, c->stream, part, B->slice_off, B->st);
+        hipLaunchKernelGGL((kb_true<W, SPW, false>), grid, blk, 0, c->stream, A, x, b, B->r, B->st, part);
+        hipLaunchKernelGGL(kb_true_finish<false>, sys, fin, 0, c->stream, part, B->slice_off, B->st, rel_tol2);
     }
 }
 
 static int enqueue_variant(hpccg_ctx *c, Batch *B, int v, const double *b, double *x, int max_iter, double rel_tol2)
 {
-    const BatchLayout &L = B->lay[v];
-    switch (kVariants[v].rows) {
-    case 64: enqueue_batch<64>(c, B, L, b, x, max_iter, rel_tol2); break;
-    case 128: enqueue_batch<128>(c, B, L, b, x, max_iter, rel_tol2); break;
-    case 256: enqueue_batch<256>(c, B, L, b, x, max_iter, rel_tol2); break;
+    switch (v) {
+    case 0: enqueue_batch<4, 1>(c, B, b, x, max_iter, rel_tol2); break;
+    case 1: enqueue_batch<8, 1>(c, B, b, x, max_iter, rel_tol2); break;
+    case 2: enqueue_batch<2, 1>(c, B, b, x, max_iter, rel_tol2); break;
+    case 3: enqueue_batch<16, 1>(c, B, b, x, max_iter, rel_tol2); break;
+    case 4: enqueue_batch<4, 2>(c, B, b, x, max_iter, rel_tol2); break;
+    case 5: enqueue_batch<8, 2>(c, B, b, x, max_iter, rel_tol2); break;
+    case 6: enqueue_batch<4, 4>(c, B, b, x, max_iter, rel_tol2); break;
+    case 7: enqueue_batch<8, 4>(c, B, b, x, max_iter, rel_tol2); break;
     default: set_error("batched: bad variant"); return HPCCG_ERR_ARG;
     }
     HPCCG_HIP_TRY(hipGetLastError());
@@ -434,7 +397,6 @@ static int enqueue_variant(hpccg_ctx *c, Batch *B, int v, const double *b, doubl
 static int batch_solve(hpccg_ctx *c, Batch *B, const double *b, double *x, double rel_tol, int max_iter,
                        int replace_every, int v, double *ms_out)
 {
-    HPCCG_TRY(build_layout(c, B, v));
     std::vector<BatchState> init(B->S);
     for (auto &s : init) {
         std::memset(&s, 0, sizeof(s));
@@ -544,6 +506,17 @@ int hpccg_batch_setup(hpccg_ctx *c, int32_t count, const int32_t *dims)
         set_error("batched: device allocation of the sliced copy failed");
         return HPCCG_ERR_NOMEM;
     }
+    B->NS = soff[count];
+    std::vector<int> ssys(std::max(B->NS, 1), 0);
+    for (int s = 0; s < count; ++s)
+        for (int i = soff[s]; i < soff[s + 1]; ++i) ssys[i] = s;
+    if (hipMalloc(&B->slice_sys, ssys.size() * sizeof(int)) != hipSuccess ||
+        hipMalloc(&B->partials, 2 * ssys.size() * sizeof(double)) != hipSuccess) {
+        free_batched(c);
+        set_error("batched: device allocation failed");
+        return HPCCG_ERR_NOMEM;
+    }
+    HPCCG_HIP_TRY(hipMemcpy(B->slice_sys, ssys.data(), ssys.size() * sizeof(int), hipMemcpyHostToDevice));
     HPCCG_HIP_TRY(hipMemcpy(B->slice_ptr, sp.data(), sp.size() * sizeof(int64_t), hipMemcpyHostToDevice));
     HPCCG_HIP_TRY(hipMemcpy(B->slice_off, soff.data(), (count + 1) * sizeof(int), hipMemcpyHostToDevice));
     for (int s = 0; s < count; ++s) {
@@ -583,8 +556,6 @@ int hpccg_batch_spmv_dev(hpccg_ctx *c, const float *x, float *y)
         set_error("no batch");
         return HPCCG_ERR_STATE;
     }
-    HPCCG_TRY(build_layout(c, B, 2));
-    const BatchLayout &L = B->lay[2];
     std::vector<BatchState> init(B->S);
     for (auto &s : init) {
         std::memset(&s, 0, sizeof(s));
@@ -592,19 +563,19 @@ int hpccg_batch_spmv_dev(hpccg_ctx *c, const float *x, float *y)
         s.rtrans = 1.0;
     }
     HPCCG_HIP_TRY(hipMemcpyAsync(B->st, init.data(), B->S * sizeof(BatchState), hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(kb_spmv<128>, dim3(L.G), dim3(128), 0, c->stream, batch_sell(B), x, y, B->row_off, L.wg_team,
-                       L.team_first, L.team_size, B->st, B->partials);
+    hipLaunchKernelGGL((kb_spmv<4, 1>), dim3((unsigned)((B->NS + 3) / 4)), dim3(256), 0, c->stream, batch_sell(B), x, y,
+                       B->st, B->partials);
     HPCCG_HIP_TRY(hipGetLastError());
     HPCCG_HIP_TRY(hipStreamSynchronize(c->stream));
     return 0;
 }
 
-int hpccg_batch_variants(int32_t *count, int32_t *rows, int32_t *occ)
+int hpccg_batch_variants(int32_t *count, int32_t *rows, int32_t *waves)
 {
     if (count) *count = kNumVariants;
     for (int v = 0; v < kNumVariants; ++v) {
-        if (rows) rows[v] = kVariants[v].rows;
-        if (occ) occ[v] = kVariants[v].occ;
+        if (rows) rows[v] = kSlice * kVariants[v].spw;
+        if (waves) waves[v] = kVariants[v].waves;
     }
     return 0;
 }
@@ -669,7 +640,7 @@ int hpccg_batch_solve_dev(hpccg_ctx *c, const double *b, double *x, double rel_t
         return HPCCG_ERR_STATE;
     }
     HPCCG_HIP_TRY(hipSetDevice(c->device));
-    const int v = variant >= 0 ? variant : (B->best_variant >= 0 ? B->best_variant : 3);
+    const int v = variant >= 0 ? variant : (B->best_variant >= 0 ? B->best_variant : 0);
     HPCCG_TRY(batch_solve(c, B, b, x, rel_tol, max_iter, replace_every, v, elapsed_ms));
     std::vector<BatchState> s(B->S);
     HPCCG_HIP_TRY(hipMemcpy(s.data(), B->st, B->S * sizeof(BatchState), hipMemcpyDeviceToHost));

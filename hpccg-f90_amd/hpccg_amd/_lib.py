@@ -377,9 +377,9 @@ class Context:
     def batch_variants():
         n = ctypes.c_int32()
         lib().hpccg_batch_variants(ctypes.byref(n), None, None)
-        rows, occ = np.zeros(n.value, np.int32), np.zeros(n.value, np.int32)
-        lib().hpccg_batch_variants(ctypes.byref(n), _ptr(rows), _ptr(occ))
-        return list(zip(rows.tolist(), occ.tolist()))
+        rows, waves = np.zeros(n.value, np.int32), np.zeros(n.value, np.int32)
+        lib().hpccg_batch_variants(ctypes.byref(n), _ptr(rows), _ptr(waves))
+        return list(zip(rows.tolist(), waves.tolist()))  # (rows per wave, waves per workgroup)
 
     def batch_autotune(self, b: DeviceArray, x: DeviceArray, probe_iters=8):
         nv = len(self.batch_variants())

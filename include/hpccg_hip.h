@@ -153,8 +153,8 @@ int hpccg_batch_setup(hpccg_ctx *ctx, int32_t count, const int32_t *dims);
 int hpccg_batch_info(hpccg_ctx *ctx, int32_t *count, int64_t *total_rows, int64_t *total_nnz,
                      int64_t *row_off);
 int hpccg_batch_spmv_dev(hpccg_ctx *ctx, const float *x, float *y);
-/* compiled launch variants: rows per workgroup tile and workgroups per CU */
-int hpccg_batch_variants(int32_t *count, int32_t *rows, int32_t *occ);
+/* compiled launch variants: rows per wave (64 per slice) and waves per workgroup */
+int hpccg_batch_variants(int32_t *count, int32_t *rows, int32_t *waves);
 /* autotuner: times every variant on this batch (probe_iters iterations each, x restored) and
  * keeps the fastest for solves with variant = -1 */
 int hpccg_batch_autotune(hpccg_ctx *ctx, const double *b, double *x, int32_t probe_iters,
