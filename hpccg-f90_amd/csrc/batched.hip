@@ -61,6 +61,7 @@ struct Batch {
     float *r = nullptr, *p = nullptr, *Ap = nullptr;
     BatchState *st = nullptr;
     double *partials = nullptr;         // [2 NS]: per-slice dot partials
+    int *nactive = nullptr;             // systems still iterating (0: every launch returns at once)
     int best_variant = -1;
     hipGraphExec_t graph = nullptr;
     const void *g_b = nullptr, *g_x = nullptr;
@@ -171,8 +172,9 @@ __device__ __forceinline__ double system_sum(const double *__restrict__ part, co
 template <int W, int SPW, bool PREP>
 __global__ __launch_bounds__(W * 64) void kb_true(const BatchSell A, const double *__restrict__ x,
                                                    const double *__restrict__ b, float *__restrict__ r,
-                                                   const BatchState *st, double *part)
+                                                   const BatchState *st, double *part, const int *nactive)
 {
+    if (!PREP && *nactive == 0) return;
     const int lane = threadIdx.x % kSlice;
     wave_slices<W, SPW>(A.NS, [&](int gs) {
         const SlicePos q = slice_pos(A, gs, lane);
@@ -200,7 +202,7 @@ __global__ __launch_bounds__(W * 64) void kb_true(const BatchSell A, const doubl
 // one workgroup per system: the true residual's norm, convergence, the iteration count
 template <bool PREP>
 __global__ __launch_bounds__(256) void kb_true_finish(const double *__restrict__ part, const int *__restrict__ slice_off,
-                                                      BatchState *st, double rel_tol2)
+                                                      BatchState *st, double rel_tol2, int *nactive)
 {
     __shared__ double red[4];
     const int s = blockIdx.x;
@@ -218,6 +220,7 @@ __global__ __launch_bounds__(256) void kb_true_finish(const double *__restrict__
         S->niters = 0;
         S->active = (a < S->tol2) ? 0 : (S->max_iter > 0 ? 1 : 0);
         S->converged = a < S->tol2;
+        if (S->active) atomicAdd(nactive, 1);
     } else {
         const int k = S->k;
         S->niters = k;
@@ -227,6 +230,7 @@ __global__ __launch_bounds__(256) void kb_true_finish(const double *__restrict__
         } else if (k >= S->max_iter) {
             S->active = 0;
         }
+        if (!S->active) atomicSub(nactive, 1);
         S->beta = a / S->rtrans;
         S->rtrans = a;
         S->need_true = 0;
@@ -237,8 +241,10 @@ __global__ __launch_bounds__(256) void kb_true_finish(const double *__restrict__
 // Top of iteration k: p = r (k == 1) or p = r + beta p.
 template <int W, int SPW>
 __global__ __launch_bounds__(W * 64) void kb_update_p(const BatchSell A, const float *__restrict__ r,
-                                                       float *__restrict__ p, const BatchState *st)
+                                                       float *__restrict__ p, const BatchState *st,
+                                                       const int *nactive)
 {
+    if (*nactive == 0) return;
     const int lane = threadIdx.x % kSlice;
     wave_slices<W, SPW>(A.NS, [&](int gs) {
         const SlicePos q = slice_pos(A, gs, lane);
@@ -252,8 +258,10 @@ __global__ __launch_bounds__(W * 64) void kb_update_p(const BatchSell A, const f
 // Ap = A p and the slice's p.Ap partial at part[gs]
 template <int W, int SPW>
 __global__ __launch_bounds__(W * 64) void kb_spmv(const BatchSell A, const float *__restrict__ p,
-                                                   float *__restrict__ Ap, const BatchState *st, double *part)
+                                                   float *__restrict__ Ap, const BatchState *st, double *part,
+                                                   const int *nactive)
 {
+    if (*nactive == 0) return;
     const int lane = threadIdx.x % kSlice;
     wave_slices<W, SPW>(A.NS, [&](int gs) {
         const SlicePos q = slice_pos(A, gs, lane);
@@ -290,8 +298,9 @@ template <int W, int SPW>
 __global__ __launch_bounds__(W * 64) void kb_update_xr(const BatchSell A, double *__restrict__ x,
                                                         float *__restrict__ r, const float *__restrict__ p,
                                                         const float *__restrict__ Ap, const BatchState *st,
-                                                        double *part)
+                                                        double *part, const int *nactive)
 {
+    if (*nactive == 0) return;
     const int lane = threadIdx.x % kSlice;
     wave_slices<W, SPW>(A.NS, [&](int gs) {
         const SlicePos q = slice_pos(A, gs, lane);
@@ -353,6 +362,7 @@ void free_batched(hpccg_ctx *c)
     hipFree(B->st);
     hipFree(B->partials);
     hipFree(B->slice_sys);
+    hipFree(B->nactive);
     delete B;
     c->ext[0] = nullptr;
 }
@@ -363,16 +373,18 @@ static void enqueue_batch(hpccg_ctx *c, Batch *B, const double *b, double *x, in
     const dim3 grid((unsigned)((B->NS + W * SPW - 1) / (W * SPW))), blk(W * 64), sys((unsigned)B->S), fin(256);
     const BatchSell A = batch_sell(B);
     double *part = B->partials;
-    hipLaunchKernelGGL((kb_true<W, SPW, true>), grid, blk, 0, c->stream, A, x, b, B->r, B->st, part);
-    hipLaunchKernelGGL(kb_true_finish<true>, sys, fin, 0, c->stream, part, B->slice_off, B->st, rel_tol2);
+    int *na = B->nactive;
+    hipLaunchKernelGGL((kb_true<W, SPW, true>), grid, blk, 0, c->stream, A, x, b, B->r, B->st, part, na);
+    hipLaunchKernelGGL(kb_true_finish<true>, sys, fin, 0, c->stream, part, B->slice_off, B->st, rel_tol2, na);
     for (int it = 0; it < max_iter; ++it) {
-        hipLaunchKernelGGL((kb_update_p<W, SPW>), grid, blk, 0, c->stream, A, B->r, B->p, B->st);
-        hipLaunchKernelGGL((kb_spmv<W, SPW>), grid, blk, 0, c->stream, A, B->p, B->Ap, B->st, part);
+        hipLaunchKernelGGL((kb_update_p<W, SPW>), grid, blk, 0, c->stream, A, B->r, B->p, B->st, na);
+        hipLaunchKernelGGL((kb_spmv<W, SPW>), grid, blk, 0, c->stream, A, B->p, B->Ap, B->st, part, na);
         hipLaunchKernelGGL(kb_pap_finish, sys, fin, 0, c->stream, part, B->slice_off, B->st);
-        hipLaunchKernelGGL((kb_update_xr<W, SPW>), grid, blk, 0, c->stream, A, x, B->r, B->p, B->Ap, B->st, part);
+        hipLaunchKernelGGL((kb_update_xr<W, SPW>), grid, blk, 0, c->stream, A, x, B->r, B->p, B->Ap, B->st, part,
+                           na);
         hipLaunchKernelGGL(kb_xr_finish, sys, fin, 0, c->stream, part, B->slice_off, B->st);
-        hipLaunchKernelGGL((kb_true<W, SPW, false>), grid, blk, 0, c->stream, A, x, b, B->r, B->st, part);
-        hipLaunchKernelGGL(kb_true_finish<false>, sys, fin, 0, c->stream, part, B->slice_off, B->st, rel_tol2);
+        hipLaunchKernelGGL((kb_true<W, SPW, false>), grid, blk, 0, c->stream, A, x, b, B->r, B->st, part, na);
+        hipLaunchKernelGGL(kb_true_finish<false>, sys, fin, 0, c->stream, part, B->slice_off, B->st, rel_tol2, na);
     }
 }
 
@@ -404,6 +416,8 @@ static int batch_solve(hpccg_ctx *c, Batch *B, const double *b, double *x, doubl
         s.max_iter = max_iter;
     }
     HPCCG_HIP_TRY(hipMemcpyAsync(B->st, init.data(), B->S * sizeof(BatchState), hipMemcpyHostToDevice, c->stream));
+    static const int zero = 0;
+    HPCCG_HIP_TRY(hipMemcpyAsync(B->nactive, &zero, sizeof(int), hipMemcpyHostToDevice, c->stream));
     const double rel_tol2 = rel_tol * rel_tol;
     // the graph bakes pointers, the variant, max_iter and rel_tol2: re-capture when they change
     if (!B->graph || B->g_b != b || B->g_x != x || B->g_variant != v || B->g_max_iter != max_iter ||
@@ -511,7 +525,8 @@ int hpccg_batch_setup(hpccg_ctx *c, int32_t count, const int32_t *dims)
     for (int s = 0; s < count; ++s)
         for (int i = soff[s]; i < soff[s + 1]; ++i) ssys[i] = s;
     if (hipMalloc(&B->slice_sys, ssys.size() * sizeof(int)) != hipSuccess ||
-        hipMalloc(&B->partials, 2 * ssys.size() * sizeof(double)) != hipSuccess) {
+        hipMalloc(&B->partials, 2 * ssys.size() * sizeof(double)) != hipSuccess ||
+        hipMalloc(&B->nactive, sizeof(int)) != hipSuccess) {
         free_batched(c);
         set_error("batched: device allocation failed");
         return HPCCG_ERR_NOMEM;
@@ -563,8 +578,10 @@ int hpccg_batch_spmv_dev(hpccg_ctx *c, const float *x, float *y)
         s.rtrans = 1.0;
     }
     HPCCG_HIP_TRY(hipMemcpyAsync(B->st, init.data(), B->S * sizeof(BatchState), hipMemcpyHostToDevice, c->stream));
+    const int all = B->S;
+    HPCCG_HIP_TRY(hipMemcpyAsync(B->nactive, &all, sizeof(int), hipMemcpyHostToDevice, c->stream));
     hipLaunchKernelGGL((kb_spmv<4, 1>), dim3((unsigned)((B->NS + 3) / 4)), dim3(256), 0, c->stream, batch_sell(B), x, y,
-                       B->st, B->partials);
+                       B->st, B->partials, B->nactive);
     HPCCG_HIP_TRY(hipGetLastError());
     HPCCG_HIP_TRY(hipStreamSynchronize(c->stream));
     return 0;
