@@ -528,8 +528,6 @@ __global__ __launch_bounds__(kFinishThreads) void kf_spmv_finish(const double *_
 }
 
 // x += alpha p (FP64), r -= alpha Ap (FP32), and the workgroup's r.r
-// XV: x is 16-byte aligned and read/written as double pairs
-template <bool XV>
 __global__ __launch_bounds__(kVecThreads) void kf_update_xr(double *__restrict__ x, float *__restrict__ r,
                                                             const float *__restrict__ p,
                                                             const float *__restrict__ Ap, int n, const FcgState *st,
@@ -540,43 +538,14 @@ __global__ __launch_bounds__(kVecThreads) void kf_update_xr(double *__restrict__
     const double alpha = st->alpha;
     int64_t i0, i1;
     int lb;
-    elem_range4(n, i0, i1, lb);
+    elem_range(n, i0, i1, lb);
     double dot = 0.0;
-    auto one = [&](float pi, float api, float ri0, double &xi) {
-        xi = fma(alpha, (double)pi, xi);
-        const float ri = (float)fma(-alpha, (double)api, (double)ri0);
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += kVecThreads) {
+        x[i] = fma(alpha, (double)p[i], x[i]);
+        const float ri = (float)fma(-alpha, (double)Ap[i], (double)r[i]);
+        r[i] = ri;
         dot = fma((double)ri, (double)ri, dot);
-        return ri;
-    };
-    vec4_loop(
-        i0, i1,
-        [&](int64_t i) {
-            const f32x4v pv = ld4(p + i), av = ld4(Ap + i);
-            f32x4v rv = ld4(r + i);
-            double xs[4];
-            if (XV) {
-                const f64x2_t a = *(const f64x2_t *)(x + i), b2 = *(const f64x2_t *)(x + i + 2);
-                xs[0] = a.x, xs[1] = a.y, xs[2] = b2.x, xs[3] = b2.y;
-            } else {
-                for (int j = 0; j < 4; ++j) xs[j] = x[i + j];
-            }
-            rv.x = one(pv.x, av.x, rv.x, xs[0]);
-            rv.y = one(pv.y, av.y, rv.y, xs[1]);
-            rv.z = one(pv.z, av.z, rv.z, xs[2]);
-            rv.w = one(pv.w, av.w, rv.w, xs[3]);
-            st4(r + i, rv);
-            if (XV) {
-                *(f64x2_t *)(x + i) = f64x2_t{xs[0], xs[1]};
-                *(f64x2_t *)(x + i + 2) = f64x2_t{xs[2], xs[3]};
-            } else {
-                for (int j = 0; j < 4; ++j) x[i + j] = xs[j];
-            }
-        },
-        [&](int64_t i) {
-            double xi = x[i];
-            r[i] = one(p[i], Ap[i], r[i], xi);
-            x[i] = xi;
-        });
+    }
     store_partial(block_sum<kVecThreads>(dot, red), partials);
 }
 
@@ -1140,17 +1109,12 @@ int hpccg_fcg_learned_dev(hpccg_ctx *c, int32_t nx, int32_t ny, int32_t nz, cons
     const int *skip = &F->st->done;
     const float *s_in = &F->st->s_in, *s_out = &F->st->s_out;
 
-    const bool x_pairs = ((uintptr_t)x % 16) == 0;
     auto iteration = [&]() {
         hipLaunchKernelGGL(kf_spmv, dim3(Gs), dim3(kSellThreads), 0, c->stream, sell16(A), F->p, F->Ap, F->st,
                            A.red_part);
         hipLaunchKernelGGL(kf_spmv_finish, dim3(1), dim3(kFinishThreads), 0, c->stream, A.red_part, Gs, F->st);
-        if (x_pairs)
-            hipLaunchKernelGGL(kf_update_xr<true>, dim3(Gv), dim3(kVecThreads), 0, c->stream, x, F->r, F->p, F->Ap,
-                               n, F->st, c->partials);
-        else
-            hipLaunchKernelGGL(kf_update_xr<false>, dim3(Gv), dim3(kVecThreads), 0, c->stream, x, F->r, F->p, F->Ap,
-                               n, F->st, c->partials);
+        hipLaunchKernelGGL(kf_update_xr, dim3(Gv), dim3(kVecThreads), 0, c->stream, x, F->r, F->p, F->Ap, n, F->st,
+                           c->partials);
         hipLaunchKernelGGL(kf_xr_finish, dim3(1), dim3(kFinishThreads), 0, c->stream, c->partials, Gv, n, F->st,
                            hrec);
         hipLaunchKernelGGL(kf_true<false>, dim3(Gt), dim3(kTrueThreads), 0, c->stream, sell16(A), x, b, F->r, F->st,

@@ -161,9 +161,7 @@ __global__ __launch_bounds__(kFinishThreads) void km_spmv_finish(const double *_
     }
 }
 
-// x = x + alpha p (FP64), r = r - alpha Ap (FP32 store), and the workgroup's recursive r.r.
-// XV: x is 16-byte aligned and read/written as double pairs.
-template <bool XV>
+// x = x + alpha p (FP64), r = r - alpha Ap (FP32 store), and the workgroup's recursive r.r
 __global__ __launch_bounds__(kVecThreads) void km_update_xr(double *__restrict__ x, float *__restrict__ r,
                                                             const float *__restrict__ p,
                                                             const float *__restrict__ Ap, int n, const MixState *st,
@@ -174,43 +172,14 @@ __global__ __launch_bounds__(kVecThreads) void km_update_xr(double *__restrict__
     const double alpha = st->alpha;
     int64_t i0, i1;
     int lb;
-    elem_range4(n, i0, i1, lb);
+    elem_range(n, i0, i1, lb);
     double dot = 0.0;
-    auto one = [&](float pi, float api, float ri0, double &xi) {
-        xi = fma(alpha, (double)pi, xi);
-        const float ri = (float)fma(-alpha, (double)api, (double)ri0);
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += kVecThreads) {
+        x[i] = fma(alpha, (double)p[i], x[i]);
+        const float ri = (float)fma(-alpha, (double)Ap[i], (double)r[i]);
+        r[i] = ri;
         dot = fma((double)ri, (double)ri, dot);
-        return ri;
-    };
-    vec4_loop(
-        i0, i1,
-        [&](int64_t i) {
-            const f32x4v pv = ld4(p + i), av = ld4(Ap + i);
-            f32x4v rv = ld4(r + i);
-            double xs[4];
-            if (XV) {
-                const f64x2_t a = *(const f64x2_t *)(x + i), b2 = *(const f64x2_t *)(x + i + 2);
-                xs[0] = a.x, xs[1] = a.y, xs[2] = b2.x, xs[3] = b2.y;
-            } else {
-                for (int j = 0; j < 4; ++j) xs[j] = x[i + j];
-            }
-            rv.x = one(pv.x, av.x, rv.x, xs[0]);
-            rv.y = one(pv.y, av.y, rv.y, xs[1]);
-            rv.z = one(pv.z, av.z, rv.z, xs[2]);
-            rv.w = one(pv.w, av.w, rv.w, xs[3]);
-            st4(r + i, rv);
-            if (XV) {
-                *(f64x2_t *)(x + i) = f64x2_t{xs[0], xs[1]};
-                *(f64x2_t *)(x + i + 2) = f64x2_t{xs[2], xs[3]};
-            } else {
-                for (int j = 0; j < 4; ++j) x[i + j] = xs[j];
-            }
-        },
-        [&](int64_t i) {
-            double xi = x[i];
-            r[i] = one(p[i], Ap[i], r[i], xi);
-            x[i] = xi;
-        });
+    }
     store_partial(block_sum<kVecThreads>(dot, red), partials);
 }
 
@@ -364,12 +333,8 @@ static int mixed_enqueue(hpccg_ctx *c, const double *b, double *x, int count)
     for (int it = 0; it < count; ++it) {
         hipLaunchKernelGGL(km_update_p, dim3(Gv), dim3(kVecThreads), 0, c->stream, m.r, m.p, n, st);
         launch_km_spmv(c, m.p, m.Ap);
-        if (((uintptr_t)x % 16) == 0)
-            hipLaunchKernelGGL(km_update_xr<true>, dim3(Gv), dim3(kVecThreads), 0, c->stream, x, m.r, m.p, m.Ap, n,
-                               st, c->partials);
-        else
-            hipLaunchKernelGGL(km_update_xr<false>, dim3(Gv), dim3(kVecThreads), 0, c->stream, x, m.r, m.p, m.Ap, n,
-                               st, c->partials);
+        hipLaunchKernelGGL(km_update_xr, dim3(Gv), dim3(kVecThreads), 0, c->stream, x, m.r, m.p, m.Ap, n, st,
+                           c->partials);
         hipLaunchKernelGGL(km_xr_finish, dim3(1), dim3(kFinishThreads), 0, c->stream, c->partials, Gv, st, hrec);
         launch_km_true<false>(c, b, x);
     }

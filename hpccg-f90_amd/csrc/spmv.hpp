@@ -34,7 +34,7 @@ constexpr int kSellWaves = 4;         // slices per SpMV workgroup (one per wave
 constexpr int kSellThreads = kSlice * kSellWaves;
 // true-residual kernels run only every few iterations and exit at once otherwise: 16 slices
 // per workgroup keep the cost of those empty launches low (16x fewer workgroups)
-constexpr int kTrueWaves = 16;
+constexpr int kTrueWaves = 4;
 constexpr int kTrueThreads = kSlice * kTrueWaves;
 constexpr int kStencilK = 27;         // slice width of the 27-point stencil: fully unrolled path
 constexpr int kGroupF64 = 2;          // FP64 copy: 2 entries per lane per load (16-byte value loads)
