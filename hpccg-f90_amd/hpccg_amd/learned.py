@@ -17,12 +17,14 @@ DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
 DEFAULT_WEIGHTS = os.path.join(DATA_DIR, "precond_weights.npy")
 
 # Training recipe: 20k (residual, error) pairs from plain FP64 CG runs on 64^3 grids with
-# sine + noise exact solutions (one random 16^3 window per pair), Adam with warm-up and cosine
-# decay, random reflections and sign flips of each window.
+# sine + noise exact solutions (one 16^3 window per pair, placed against a domain face with
+# probability 2/3 per axis), 6000 Adam steps of 32 windows with warm-up and cosine decay,
+# random reflections and sign flips of each window.
 # Several initialisations are trained on the same pairs and the one whose flexible CG needs the
-# fewest iterations on a validation system (its own seed, never used for evaluation) is kept.
-RECIPE = {"n": 64, "npairs": 20000, "crop": 16, "data_seed": 21, "steps": 3000, "batch": 32, "lr": 2e-3,
-          "candidates": 6, "first_seed": 3, "val_n": 40, "val_seed": 4242}
+# fewest iterations on a 128^3 validation system (its own seed, never used for evaluation) is
+# kept: on 40^3 most candidates tie, while the iteration counts of larger grids differ.
+RECIPE = {"n": 64, "npairs": 20000, "crop": 16, "data_seed": 21, "steps": 6000, "batch": 32, "lr": 2e-3,
+          "candidates": 6, "first_seed": 3, "val_n": 128, "val_seed": 4242}
 
 
 def validation_iters(ctx, weights, n, seed, max_iter=400):
