@@ -199,8 +199,9 @@ int hpccg_cnn_backward(hpccg_ctx *ctx, int32_t nx, int32_t ny, int32_t nz, int32
 int hpccg_cnn_make_dataset(hpccg_ctx *ctx, int32_t n, int32_t npairs, int32_t crop, uint64_t seed);
 /* Pair i of the training set (crop^3 floats each); margins6 (may be NULL) = loss-mask margins. */
 int hpccg_cnn_dataset_pair(hpccg_ctx *ctx, int32_t i, float *r, float *e, int32_t *margins6);
-/* `steps` Adam steps (warm-up, cosine decay) of the A-norm error loss
- * ||mask.A(z - e)||^2 / ||mask.A e||^2 on batches of random reflected crops; one captured graph
+/* `steps` Adam steps (warm-up, cosine decay) of the residual loss
+ * ||mask.A(z - e)||^2 / ||mask.A e||^2 (A e = r on the mask, so this is ||A z - r||^2 / ||r||^2)
+ * on batches of random reflected crops; one captured graph
  * per step.  loss_hist[steps] may be NULL; elapsed_ms = device time of all steps. */
 int hpccg_cnn_train(hpccg_ctx *ctx, int32_t steps, int32_t batch, float lr, uint64_t seed,
                     float *loss_hist, double *elapsed_ms);
