@@ -144,7 +144,7 @@ __global__ __launch_bounds__(kVecThreads) void k_cg_update_r(double *__restrict_
     elem_range(n, i0, i1, lb);
     double dot = 0.0;
     for (int64_t i = i0 + threadIdx.x; i < i1; i += kVecThreads) {
-        const double t = nalpha * Ap[i];
+        const double t = nalpha * __builtin_nontemporal_load(Ap + i);
         const double ri = r[i] + t;
         r[i] = ri;
         dot = fma(ri, ri, dot);
@@ -185,7 +185,7 @@ __global__ __launch_bounds__(kVecThreads) void k_cg_update_p(const double *__res
     for (int64_t i = i0 + threadIdx.x; i < i1; i += kVecThreads) {
         const double pi = p[i];
         const double tx = alpha * pi;
-        x[i] = x[i] + tx;
+        __builtin_nontemporal_store(x[i] + tx, x + i);  // x is next read an iteration later
         const double tp = beta * pi;
         p[i] = r[i] + tp;
     }
