@@ -76,6 +76,29 @@ def test_cg_mixed_device_api_matches_host_api(ctx):
     assert res2["niters"] == nit and np.array_equal(xd.numpy(), xh)
 
 
+def test_cg_mixed_forms_a_true_residual_after_a_1e5_fall(ctx):
+    """Without periodic replacement (replace_every=0), the FP32 recursive residual alone would
+    drift from b - A x by ~1e-7 of ||b||.  So the solver forms the FP64 true residual again once
+    the recursive norm has fallen 1e5-fold since the last one: at least one true residual lies
+    strictly between Prepare and the final iteration.  The solve still meets 1e-10, within a
+    few iterations of FP64 CG."""
+    grid = (48, 48, 48)
+    A = oracle.Problem(*grid)
+    b = oracle.spmv(A, oracle.xstar_sines(*grid, 21))
+    normb = math.sqrt(oracle.ddot1(b))
+    ctx.upload(A.rowindex, A.colindex, A.values)
+    x, nit, tn, hr, ht, st = ctx.cg_mixed(b, max_iter=1000, tolerance=1e-10 * normb, replace_every=0)
+    assert st == 0
+    inner = [j for j in range(1, nit) if ht[j] > 0]
+    assert inner, (nit, ht[:nit + 1])
+    j = inner[0]
+    assert hr[j] <= 1e-5 * normb * 1.0000001  # it fired on the 1e5 fall of the recursive norm
+    rel = math.sqrt(oracle.ddot1(b - oracle.spmv(A, x))) / normb
+    assert rel <= 1e-10, rel
+    _, nit64, _, _, _ = oracle.cg(A, b, max_iter=1000, tolerance=1e-10 * normb)
+    assert nit <= nit64 + 3, (nit, nit64)
+
+
 def test_cg_mixed_edges(ctx):
     A = oracle.Problem(10, 9, 8)
     ctx.upload(A.rowindex, A.colindex, A.values)
