@@ -8,7 +8,7 @@ The stopping test is the reference's (recursive residual, numerics.F90:109), wit
 1e-10 * ||b||; the FP64 true residual ||b - A x|| / ||b|| of every result is reported.
 
     python bench.py [--steps K] [--warmup W] [--grid N] [--no-cpu] [--no-learned] [--no-batched]
-                    [--spmv-only] [--gpus 1]
+                    [--spmv-only] [--network-only] [--gpus 1]
 
 Prints one JSON line (last line of stdout).
 """
@@ -105,6 +105,25 @@ def measure_roofline(ctx, xs, n, nnz, n1, reps):
     return roofline
 
 
+def measure_network(ctx, grid, reps):
+    """z = net(u) on a random grid^3 input (the fused inference the flexible CG launches):
+    30 warm-up applications, then `reps` timed ones (synchronous calls, host clock)."""
+    import numpy as np
+    import hpccg_amd
+    nvox = grid[0] * grid[1] * grid[2]
+    u = hpccg_amd.DeviceArray.from_numpy(ctx, np.random.default_rng(1).standard_normal(nvox).astype(np.float32))
+    z = hpccg_amd.DeviceArray(ctx, nvox, np.float32)
+    for _ in range(30):
+        ctx.cnn_apply_dev(grid, u, 1.0, z)
+    t = time.perf_counter()
+    for _ in range(reps):
+        ctx.cnn_apply_dev(grid, u, 1.0, z)
+    ms = (time.perf_counter() - t) / reps * 1e3
+    flop = 2.0 * 27 * (16 + 16 * 16 + 16) * nvox
+    return {"grid": list(grid), "ms_per_application": round(ms, 4), "tflops": round(flop / (ms * 1e-3) / 1e12, 1),
+            "bf16_peak_tflops": 2500.0}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,6 +135,8 @@ def main():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--retrain", action="store_true", help="train the preconditioner even if weights are stored")
+    ap.add_argument("--network-only", action="store_true",
+                    help="only time the fused network inference on a grid^3 input (for counter runs)")
     ap.add_argument("--spmv-only", action="store_true",
                     help="only build the matrix and time the SpMV and the copy (for counter runs)")
     args = ap.parse_args()
@@ -150,6 +171,10 @@ def main():
                 train_info = dict(json.load(f), source="stored weights (trained on the GPU by hpccg_amd.learned.train)")
     if weights is not None:
         ctx.cnn_set_weights(weights)
+    if args.network_only:
+        print(json.dumps({"network": measure_network(ctx, grid, args.spmv_reps)}), flush=True)
+        ctx.close()
+        return
     t0 = time.perf_counter()
     ctx.generate(*grid)
     info = ctx.matrix_info()
