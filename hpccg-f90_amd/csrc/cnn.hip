@@ -591,8 +591,8 @@ __global__ __launch_bounds__(kVecThreads) void kf_dots(const float *__restrict__
     vec4_loop(
         i0, i1,
         [&](int64_t i) {
-            const f32x4v zv = ld4(z + i), rv = ld4(r + i);
-            const f32x4v av = PREP ? f32x4v{0.f, 0.f, 0.f, 0.f} : ld4(Ap + i);
+            const f32x4v zv = ld4(z + i), rv = ld4_nt(r + i);
+            const f32x4v av = PREP ? f32x4v{0.f, 0.f, 0.f, 0.f} : ld4_nt(Ap + i);
             one(zv.x, av.x, rv.x);
             one(zv.y, av.y, rv.y);
             one(zv.z, av.z, rv.z);
@@ -648,7 +648,7 @@ __global__ __launch_bounds__(kVecThreads) void kf_update_p(const float *__restri
     vec4_loop(
         i0, i1,
         [&](int64_t i) {
-            const f32x4v zv = ld4(z + i);
+            const f32x4v zv = ld4_nt(z + i);  // streamed: keeps p, which the SpMV gathers next, cached
             f32x4v pv = FIRST ? zv : ld4(p + i);
             pv.x = one(zv.x, pv.x);
             pv.y = one(zv.y, pv.y);

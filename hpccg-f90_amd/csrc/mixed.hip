@@ -119,7 +119,7 @@ __global__ __launch_bounds__(kVecThreads) void km_update_p(const float *__restri
     vec4_loop(
         i0, i1,
         [&](int64_t i) {
-            const f32x4v rv = ld4(r + i);
+            const f32x4v rv = ld4_nt(r + i);  // streamed: keeps p, which the SpMV gathers next, cached
             f32x4v pv = first ? rv : ld4(p + i);
             pv.x = one(rv.x, pv.x);
             pv.y = one(rv.y, pv.y);

@@ -291,6 +291,9 @@ __device__ __forceinline__ void vec4_loop(int64_t i0, int64_t i1, F4 body4, F1 b
 }
 typedef float f32x4v __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ f32x4v ld4(const float *p) { return *(const f32x4v *)p; }
+// streamed operand (read once here, next used after the matrix has passed through the caches):
+// non-temporal, so that it does not displace the vector the next SpMV gathers
+__device__ __forceinline__ f32x4v ld4_nt(const float *p) { return __builtin_nontemporal_load((const f32x4v *)p); }
 __device__ __forceinline__ void st4(float *p, f32x4v v) { *(f32x4v *)p = v; }
 
 // Launch shapes (host).  SpMV kernels launch one wave per slice, kSellWaves slices per
