@@ -250,7 +250,7 @@ __global__ __launch_bounds__(W * 64) void kb_update_p(const BatchSell A, const f
         const SlicePos q = slice_pos(A, gs, lane);
         const BatchState *S = st + q.s;
         if (!S->active || !q.valid) return;
-        const float ri = r[q.gr];
+        const float ri = __builtin_nontemporal_load(r + q.gr);  // keeps p, which the SpMV gathers next, cached
         p[q.gr] = S->k == 1 ? ri : (float)fma(S->beta, (double)p[q.gr], (double)ri);
     });
 }
