@@ -38,7 +38,6 @@ struct BatchState {
     int converged;
     int breakdown;
     int replace_every, max_iter;
-    unsigned counter[4];
 };
 
 struct Batch {

@@ -730,7 +730,7 @@ int hpccg_create(hpccg_ctx **out, int device)
         hpccg_destroy(c);
         return HPCCG_ERR_HIP;
     }
-    // counters start at zero (a fill kernel would do as well; this runs once, never captured)
+    // the control block starts zeroed (every solve rewrites it before use)
     CgState zero{};
     memcpy(c->state_h, &zero, sizeof(zero));
     if (hipMemcpy(c->state, c->state_h, sizeof(CgState), hipMemcpyHostToDevice) != hipSuccess) {

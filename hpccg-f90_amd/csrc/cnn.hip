@@ -433,7 +433,6 @@ struct FcgState {
     double rr_true;
     int k, stop_at, max_iter, replace_every, need_true, converged, breakdown, done;
     float s_in, s_out;  // network input / output scales (1/rms(r), rms(r))
-    unsigned counter[4];
 };
 
 // r = b - A x (FP32 store, FP64 arithmetic) and the workgroup's share of r.r

@@ -82,37 +82,6 @@ __device__ __forceinline__ double block_sum(double v, double *red)
     return t;
 }
 
-__device__ __forceinline__ double load_agent(const double *p)
-{
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Single-pass grid reduction.  Every block calls this with its block total (valid in thread
-// 0).  Returns true in exactly one block -- the last to arrive -- whose threads then all hold
-// the grid total in *total, summed in a fixed order over partials[0..G) (independent of
-// arrival order).  The counter is reset for the next use.
-template <int THREADS>
-__device__ __forceinline__ bool grid_sum(double block_total, double *partials, unsigned *counter,
-                                         int G, int logical_b, double *red, double *total)
-{
-    __shared__ int is_last;
-    if (threadIdx.x == 0) {
-        partials[logical_b] = block_total;
-        __threadfence();
-        const unsigned old = atomicAdd(counter, 1u);
-        is_last = (old == (unsigned)G - 1u);
-    }
-    __syncthreads();
-    if (!is_last) return false;
-    __threadfence();
-    double s = 0.0;
-    for (int i = threadIdx.x; i < G; i += THREADS) s += load_agent(partials + i);
-    s = block_sum<THREADS>(s, red);
-    *total = s;
-    if (threadIdx.x == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return true;
-}
-
 // Device-wide launch facts, queried once per context.
 struct DeviceInfo {
     int num_cu = 256;

@@ -24,8 +24,6 @@ struct CgState {
     int stop_at;       // first iteration that does not run (max_iter+1 or lagged test)
     int max_iter;
     int breakdown;     // p.Ap was zero or non-finite
-    unsigned counter[4];
-    int pad[4];
     double rr;         // ||r_{k-1}||^2, the last recorded r.r
 };
 

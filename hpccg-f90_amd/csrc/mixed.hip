@@ -39,7 +39,6 @@ struct MixState {
     int converged;
     int breakdown;
     int inexact16;
-    unsigned counter[4];
 };
 
 // ---- kernels --------------------------------------------------------------------------------
