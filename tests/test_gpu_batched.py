@@ -48,7 +48,7 @@ def test_batch_spmv_bitwise_per_system(ctx):
     assert info["nnz"] == sum((3 * a - 2) * (3 * b - 2) * (3 * c - 2) for a, b, c in dims)
 
 
-@pytest.mark.parametrize("variant", [-1, 0, 2, 4])
+@pytest.mark.parametrize("variant", [-1, 0, 2, 4, 7])
 def test_batch_solve_every_system_meets_tolerance(ctx, variant):
     dims = oracle.batch_dims(12, 8, 40, 99)
     info, xs, b = _batch(ctx, dims, 1000)
