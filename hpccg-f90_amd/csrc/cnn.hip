@@ -204,8 +204,8 @@ static void forward_fused(hpccg_ctx *c, const float *in, const float *s_in, floa
 {
     CnnNet *N = net_of(c);
     const int tx = (g.nx + FX - 1) / FX, ty = (g.ny + FY - 1) / FY;
-    // about one workgroup per CU (registers admit one 8-wave workgroup), z-chunks of at least
-    // 8 planes (each chunk re-reads 6 planes of halo)
+    // about one workgroup per CU (registers admit one 12-wave workgroup per CU), z-chunks of at
+    // least 8 planes (each chunk re-reads 6 planes of halo)
     const int want = std::max(1, (c->info.num_cu + tx * ty - 1) / (tx * ty));
     const int nzc = std::max(1, std::min(want, (g.nz + 7) / 8));
     const int zchunk = (g.nz + nzc - 1) / nzc;
