@@ -597,8 +597,8 @@ int hpccg_batch_variants(int32_t *count, int32_t *rows, int32_t *waves)
 }
 
 // Times every compiled launch variant on this batch (solves of `probe_iters` iterations from
-// the given b and x, x restored afterwards), keeps the fastest for variant = -1 solves, and
-// returns it with the per-variant times (ms).
+// the given b and x, x restored afterwards; see the score below), keeps the fastest for
+// variant = -1 solves, and returns it with the per-variant scores (ms).
 int hpccg_batch_autotune(hpccg_ctx *c, const double *b, double *x, int32_t probe_iters, int32_t *best,
                          double *variant_ms)
 {
@@ -611,25 +611,36 @@ int hpccg_batch_autotune(hpccg_ctx *c, const double *b, double *x, int32_t probe
     double *x0 = nullptr;
     HPCCG_HIP_TRY(hipMalloc(&x0, B->E * sizeof(double)));
     HPCCG_HIP_TRY(hipMemcpyAsync(x0, x, B->E * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-    int bi = -1;
-    double bt = 1e300;
-    for (int v = 0; v < kNumVariants; ++v) {
+    // Score = time of probe_iters working iterations + time of probe_iters iterations after
+    // every system has converged (rel_tol 1e100 stops them all at Prepare).  A solve captures
+    // max_iter iterations, so the idle ones past the last convergence count too, and the
+    // variants with more workgroups launch them more slowly.
+    auto probe = [&](int v, double rel_tol, double *out) -> int {
         double t = 0.0, ms = 0.0;
         for (int rep = 0; rep < 3; ++rep) {
             HPCCG_HIP_TRY(hipMemcpyAsync(x, x0, B->E * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-            const int st = batch_solve(c, B, b, x, 0.0, probe_iters, 0, v, &ms);
-            if (st) {
-                hipFree(x0);
-                return st;
-            }
-            if (rep > 0) t += ms;  // first run captures and warms
+            HPCCG_TRY(batch_solve(c, B, b, x, rel_tol, probe_iters, 0, v, &ms));
+            if (rep > 0) t += ms;  // the first run captures and warms
         }
-        t /= 2;
-        if (variant_ms) variant_ms[v] = t;
-        if (t < bt) {
-            bt = t;
-            bi = v;
-        }
+        *out = t / 2;
+        return 0;
+    };
+    std::vector<double> score(kNumVariants, 0.0);
+    int st = 0;
+    for (int v = 0; v < kNumVariants && st == 0; ++v) st = probe(v, 0.0, &score[v]);
+    for (int v = 0; v < kNumVariants && st == 0; ++v) {
+        double idle = 0.0;
+        st = probe(v, 1e100, &idle);
+        score[v] += idle;
+    }
+    if (st) {
+        hipFree(x0);
+        return st;
+    }
+    int bi = 0;
+    for (int v = 0; v < kNumVariants; ++v) {
+        if (variant_ms) variant_ms[v] = score[v];
+        if (score[v] < score[bi]) bi = v;
     }
     HPCCG_HIP_TRY(hipMemcpyAsync(x, x0, B->E * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     HPCCG_HIP_TRY(hipStreamSynchronize(c->stream));

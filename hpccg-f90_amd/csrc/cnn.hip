@@ -606,40 +606,38 @@ __global__ __launch_bounds__(kVecThreads) void kf_dots(const float *__restrict__
     }
 }
 
-// beta = -(z.Ap)/(p.Ap) (flexible CG, Notay 2000), rz = z.r; p = z + beta p follows in
-// kf_update_p.  PREP: rz = z.r only (first direction p = z).
-template <bool PREP>
-__global__ __launch_bounds__(kFinishThreads) void kf_dots_finish(const double *__restrict__ partials, int nparts,
-                                                                 FcgState *st)
+// beta and p.  Every workgroup sums the z.Ap / z.r partials of kf_dots itself (two per kf_dots
+// workgroup, 2,048 values at 256 CUs; the same fixed order in every workgroup, so all agree) and forms beta = -(z.Ap)/(p.Ap)
+// (flexible CG, Notay 2000) -- no separate finish launch.  Workgroup 0 records beta and
+// rz = z.r.  A z that is not a descent direction (z.r <= 0 or not finite) stops the loop for
+// the host's fallback, and no workgroup touches p.  FIRST (Prepare): rz only, p = z.
+template <bool FIRST>
+__global__ __launch_bounds__(kVecThreads) void kf_update_p(const float *__restrict__ z, float *__restrict__ p, int n,
+                                                           FcgState *st, const double *__restrict__ partials,
+                                                           int nparts)
 {
-    __shared__ double red[kFinishThreads / kWave];
-    if (!PREP && st->k >= st->stop_at) return;
-    if (PREP && st->stop_at <= 1) return;
+    __shared__ double red[kVecThreads / kWave];
+    if (!FIRST && st->k >= st->stop_at) return;
+    if (FIRST && st->stop_at <= 1) return;
     double a = 0.0, b = 0.0;
-    for (int i = threadIdx.x; i < nparts; i += kFinishThreads) {
+    for (int i = threadIdx.x; i < nparts; i += kVecThreads) {
         a += partials[2 * i];
         b += partials[2 * i + 1];
     }
-    a = block_sum<kFinishThreads>(a, red);
-    b = block_sum<kFinishThreads>(b, red);
-    if (threadIdx.x == 0) {
-        st->beta = PREP ? 0.0 : -a / st->pap;
+    a = block_sum<kVecThreads>(a, red);
+    b = block_sum<kVecThreads>(b, red);
+    const double beta = FIRST ? 0.0 : -a / st->pap;
+    const bool breakdown = !(b > 0.0) || !isfinite(b) || !isfinite(beta);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->beta = beta;
         st->rz = b;
-        if (!(b > 0.0) || !isfinite(b) || !isfinite(st->beta)) {  // z not a descent direction
+        if (breakdown) {
             st->breakdown = 1;
             st->stop_at = st->k;
             st->done = 1;
         }
     }
-}
-
-template <bool FIRST>
-__global__ __launch_bounds__(kVecThreads) void kf_update_p(const float *__restrict__ z, float *__restrict__ p, int n,
-                                                           const FcgState *st)
-{
-    if (!FIRST && st->k >= st->stop_at) return;
-    if (FIRST && st->stop_at <= 1) return;
-    const double beta = st->beta;
+    if (breakdown) return;
     int64_t i0, i1;
     int lb;
     elem_range4(n, i0, i1, lb);
@@ -1123,8 +1121,8 @@ int hpccg_fcg_learned_dev(hpccg_ctx *c, int32_t nx, int32_t ny, int32_t nz, cons
         forward_fused(c, F->r, s_in, F->z, s_out, g, skip);
         hipLaunchKernelGGL(kf_dots<false>, dim3(Gv), dim3(kVecThreads), 0, c->stream, F->z, F->Ap, F->r, n, F->st,
                            c->partials);
-        hipLaunchKernelGGL(kf_dots_finish<false>, dim3(1), dim3(kFinishThreads), 0, c->stream, c->partials, Gv, F->st);
-        hipLaunchKernelGGL(kf_update_p<false>, dim3(Gv), dim3(kVecThreads), 0, c->stream, F->z, F->p, n, F->st);
+        hipLaunchKernelGGL(kf_update_p<false>, dim3(Gv), dim3(kVecThreads), 0, c->stream, F->z, F->p, n, F->st,
+                           c->partials, Gv);
     };
 
     // graphs of 1, 2, 4, ... iterations, launched in chunks that never cross a stagnation
@@ -1158,8 +1156,8 @@ int hpccg_fcg_learned_dev(hpccg_ctx *c, int32_t nx, int32_t ny, int32_t nz, cons
     forward_fused(c, F->r, s_in, F->z, s_out, g, skip);
     hipLaunchKernelGGL(kf_dots<true>, dim3(Gv), dim3(kVecThreads), 0, c->stream, F->z, F->Ap, F->r, n, F->st,
                        c->partials);
-    hipLaunchKernelGGL(kf_dots_finish<true>, dim3(1), dim3(kFinishThreads), 0, c->stream, c->partials, Gv, F->st);
-    hipLaunchKernelGGL(kf_update_p<true>, dim3(Gv), dim3(kVecThreads), 0, c->stream, F->z, F->p, n, F->st);
+    hipLaunchKernelGGL(kf_update_p<true>, dim3(Gv), dim3(kVecThreads), 0, c->stream, F->z, F->p, n, F->st,
+                       c->partials, Gv);
     HPCCG_HIP_TRY(hipGetLastError());
 
     bool fallback = false;

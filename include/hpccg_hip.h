@@ -155,8 +155,9 @@ int hpccg_batch_info(hpccg_ctx *ctx, int32_t *count, int64_t *total_rows, int64_
 int hpccg_batch_spmv_dev(hpccg_ctx *ctx, const float *x, float *y);
 /* compiled launch variants: rows per wave (64 per slice) and waves per workgroup */
 int hpccg_batch_variants(int32_t *count, int32_t *rows, int32_t *waves);
-/* autotuner: times every variant on this batch (probe_iters iterations each, x restored) and
- * keeps the fastest for solves with variant = -1 */
+/* autotuner: times every variant on this batch (x restored) and keeps the fastest for solves
+ * with variant = -1.  variant_ms[v] = ms of probe_iters working iterations + ms of probe_iters
+ * iterations after every system has converged (a solve's graph holds max_iter iterations). */
 int hpccg_batch_autotune(hpccg_ctx *ctx, const double *b, double *x, int32_t probe_iters,
                          int32_t *best, double *variant_ms);
 /* every system to ||b_s - A_s x_s|| <= rel_tol ||b_s|| (FP64 true residual) or max_iter;
