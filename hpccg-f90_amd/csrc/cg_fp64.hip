@@ -41,7 +41,7 @@ __device__ __forceinline__ double sell_spmv_f64(const SellRef<double> &A, const 
 {
     int64_t row;
     bool valid;
-    const double sum = sell_rowsum_of_thread<kGroupF64>(A, x, row, valid);
+    const double sum = sell_rowsum_of_thread<kGroupF64, double, double, kSellWaves, true>(A, x, row, valid);
     if (!valid) return 0.0;
     __builtin_nontemporal_store(sum, y + row);  // streamed out: keeps x's lines in L2
     return want_dot ? x[row] * sum : 0.0;

@@ -155,11 +155,27 @@ struct SellRef {
 
 // One wave per slice, WAVES slices per workgroup: the FP64 row sum of this thread's row (0 past
 // the last row or slice), with the row and whether it exists.
-template <int G, typename VT, typename XT, int WAVES = kSellWaves>
+// Workgroup -> slice-group numbering.  The hardware deals workgroups to the 8 XCDs round-robin
+// (workgroup b runs on XCD b % 8).  Renumbered, each XCD takes runs of kXcdRun consecutive
+// slice groups (256 x 4 slices x 64 rows = 65,536 rows: one 256 x 256 plane of a 27-point
+// grid), so the x lines a group gathers are mostly those its XCD's L2 fetched for the groups
+// just before it.  Workgroups past the last whole span of 8 runs keep their number.  Measured
+// on the FP64 SpMV at 256^3: 1.000 -> 0.973 ms (runs of 64: no change; of 2,048 or more: slower).
+constexpr int kXcdRun = 256;
+__device__ __forceinline__ int sell_block()
+{
+    const int b = blockIdx.x, span = kNumXcd * kXcdRun;
+    if (b >= ((int)gridDim.x / span) * span) return b;
+    return (b / span) * span + (b % kNumXcd) * kXcdRun + (b / kNumXcd) % kXcdRun;
+}
+
+// XRUN: number the workgroups with sell_block() (the FP64 kernels; the FP16 ones measured
+// slower with it: mixed CG 49.9 -> 51.0 ms, flexible CG 26.7 -> 27.3 ms at 256^3)
+template <int G, typename VT, typename XT, int WAVES = kSellWaves, bool XRUN = false>
 __device__ __forceinline__ double sell_rowsum_of_thread(const SellRef<VT> &A, const XT *__restrict__ x, int64_t &row,
                                                         bool &valid)
 {
-    const int s = blockIdx.x * WAVES + __builtin_amdgcn_readfirstlane(threadIdx.x / kSlice);
+    const int s = (XRUN ? sell_block() : (int)blockIdx.x) * WAVES + __builtin_amdgcn_readfirstlane(threadIdx.x / kSlice);
     const int lane = threadIdx.x % kSlice;
     row = (int64_t)s * kSlice + lane;
     valid = s < A.nslices && row < A.nrow;
