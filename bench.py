@@ -91,10 +91,17 @@ def measure_roofline(ctx, xs, n, nnz, n1, reps):
     if os.path.exists(tf):
         with open(tf) as f:
             traffic = json.load(f).get("traffic_bytes")
+    # the device copy stores one-byte column codes instead of int32 columns when the matrix has
+    # few distinct offsets col - row: the bytes that format must move, and their rate
+    fmt = ctx.matrix_format()
+    Bf = spmv_bytes(n, nnz, s_idx=1 if fmt["ncodes"] else 4)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "k_cg_spmv<true> (FP64 sliced-ELL SpMV of the CSR + p.Ap partials)",
-                "algorithmic_bytes": B, "avg_ms": round(spmv_ms, 5)}
+                "algorithmic_bytes": B, "avg_ms": round(spmv_ms, 5),
+                "column_codes": fmt["ncodes"], "format_bytes": Bf,
+                "format_GBs": round(Bf / (spmv_ms * 1e-3) / 1e9, 1),
+                "format_frac": round(Bf / (spmv_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     cb = min(B // 2, 2 ** 31) // 16 * 16
     ca = hpccg_amd.DeviceArray(ctx, cb // 8)
     cbuf = hpccg_amd.DeviceArray(ctx, cb // 8)

@@ -539,7 +539,7 @@ int hpccg_batch_setup(hpccg_ctx *c, int32_t count, const int32_t *dims)
         hipLaunchKernelGGL((k_csr_to_sell<kGroupF16, __half, __half>), dim3((unsigned)((threads + 255) / 256)),
                            dim3(256), 0, c->stream, B->rowindex + B->row_off_h[s], B->colindex, B->values,
                            B->slice_ptr + soff[s], B->scol, B->sval, B->row_off_h[s + 1] - B->row_off_h[s], ns,
-                           nullptr);
+                           nullptr, nullptr, nullptr, 0);
     }
     HPCCG_HIP_TRY(hipGetLastError());
     HPCCG_HIP_TRY(hipStreamSynchronize(c->stream));

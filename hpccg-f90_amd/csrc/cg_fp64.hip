@@ -852,14 +852,78 @@ static void free_matrix(hpccg_ctx *c)
 {
     drop_graphs(c);
     Matrix &A = c->A;
-    for (void *p : {(void *)A.rowindex, (void *)A.colindex, (void *)A.values, (void *)A.slice_ptr, (void *)A.scol64,
-                    (void *)A.sval64, (void *)A.scol16, (void *)A.sval16, (void *)A.red_part})
+    for (void *p : {(void *)A.rowindex, (void *)A.colindex, (void *)A.values, (void *)A.slice_ptr, (void *)A.code,
+                    (void *)A.ctab, (void *)A.scol64, (void *)A.sval64, (void *)A.scol16, (void *)A.sval16,
+                    (void *)A.red_part})
         hipFree(p);
     A = Matrix{};
 }
 
+// Distinct offsets (col - 1) - row of a 1-based CSR, collected into an open-addressed set of
+// kOffsetSlots slots (initialised to kNoOffset), one thread per row.  *count is the number
+// inserted; the kernel stops inserting once it exceeds kMaxCodes (the copy then keeps int32
+// columns), and a full set raises *count to kOffsetSlots.
+__global__ void k_column_offsets(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col, int nrow,
+                                 int32_t *set, int *count)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nrow) return;
+    const int k1 = rowptr[t + 1] - 1;
+    for (int k = rowptr[t] - 1; k < k1; ++k) {
+        const int32_t off = (col[k] - 1) - t;
+        uint32_t h = ((uint32_t)off * 2654435761u) >> 20;
+        int probe = 0;
+        for (; probe < kOffsetSlots; ++probe, h = (h + 1) & (kOffsetSlots - 1)) {
+            int32_t cur = __atomic_load_n(set + h, __ATOMIC_RELAXED);
+            if (cur == off) break;
+            if (cur != kNoOffset) continue;
+            if (__atomic_load_n(count, __ATOMIC_RELAXED) > kMaxCodes) return;
+            cur = atomicCAS(set + h, kNoOffset, off);
+            if (cur == kNoOffset) {
+                atomicAdd(count, 1);
+                break;
+            }
+            if (cur == off) break;
+        }
+        if (probe == kOffsetSlots) {
+            atomicMax(count, kOffsetSlots);
+            return;
+        }
+    }
+}
+
+// The sorted table of the distinct offsets col - row of the device CSR, when there are at
+// most kMaxCodes of them (tab empty otherwise).  Collected on the device in a hash set.
+static hipError_t column_offset_table(hpccg_ctx *c, std::vector<int32_t> &tab)
+{
+    Matrix &A = c->A;
+    tab.clear();
+    if (A.nrow == 0 || A.nnz == 0) return hipSuccess;
+    int32_t *set = nullptr;  // kOffsetSlots slots, then the count
+    hipError_t e = hipMalloc(&set, (kOffsetSlots + 1) * sizeof(int32_t));
+    if (e != hipSuccess) return e;
+    std::vector<int32_t> h(kOffsetSlots + 1);
+    e = hipMemsetD32Async(set, kNoOffset, kOffsetSlots, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(set + kOffsetSlots, 0, sizeof(int32_t), c->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_column_offsets, dim3((unsigned)((A.nrow + 255) / 256)), dim3(256), 0, c->stream,
+                           A.rowindex, A.colindex, A.nrow, set, set + kOffsetSlots);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), set, h.size() * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipFree(set);
+    if (e != hipSuccess || h[kOffsetSlots] > kMaxCodes) return e;
+    for (int i = 0; i < kOffsetSlots; ++i)
+        if (h[i] != kNoOffset) tab.push_back(h[i]);
+    std::sort(tab.begin(), tab.end());
+    if ((int)tab.size() > kMaxCodes) tab.clear();
+    return hipSuccess;
+}
+
 // After the CSR is on the device: row statistics, the FP64 sliced-ELL copy every FP64 SpMV
-// reads, and the reduction scratch of the one-wave-per-slice kernels.
+// reads (column-coded when the matrix has at most kMaxCodes distinct offsets col - row), and
+// the reduction scratch of the one-wave-per-slice kernels.
 static int finish_matrix_setup(hpccg_ctx *c, const int32_t *rowindex_host)
 {
     Matrix &A = c->A;
@@ -871,27 +935,52 @@ static int finish_matrix_setup(hpccg_ctx *c, const int32_t *rowindex_host)
     A.nslices = (n + kSlice - 1) / kSlice;
     std::vector<int64_t> sp(A.nslices + 1);
     A.slots = sell_slice_ptr(rowindex_host, n, 0, sp.data());
+    std::vector<int32_t> tab;
+    hipError_t e = column_offset_table(c, tab);
+    if (e != hipSuccess) {
+        free_matrix(c);
+        set_error("matrix setup: %s", hipGetErrorString(e));
+        return HPCCG_ERR_HIP;
+    }
+    const bool coded = !tab.empty();
+    const int64_t cbytes = code_bytes(A.slots, A.nslices);
     const int G = sell_grid(A);
+    int *flag = nullptr;
     if (hipMalloc(&A.slice_ptr, sp.size() * sizeof(int64_t)) != hipSuccess ||
-        hipMalloc(&A.scol64, std::max<int64_t>(A.slots, 1) * sizeof(int32_t)) != hipSuccess ||
+        (coded && hipMalloc(&A.code, std::max<int64_t>(cbytes, 4)) != hipSuccess) ||
+        (coded && hipMalloc(&A.ctab, kMaxCodes * sizeof(int32_t)) != hipSuccess) ||
+        (!coded && hipMalloc(&A.scol64, std::max<int64_t>(A.slots, 1) * sizeof(int32_t)) != hipSuccess) ||
         hipMalloc(&A.sval64, std::max<int64_t>(A.slots, 1) * sizeof(double)) != hipSuccess ||
-        hipMalloc(&A.red_part, (size_t)G * sizeof(double)) != hipSuccess) {
+        hipMalloc(&A.red_part, (size_t)G * sizeof(double)) != hipSuccess || hipMalloc(&flag, sizeof(int)) != hipSuccess) {
+        hipFree(flag);
         free_matrix(c);
         set_error("matrix setup: device allocation of the sliced copy failed (%lld slots)", (long long)sp.back());
         return HPCCG_ERR_NOMEM;
     }
-    hipError_t e = hipMemcpyAsync(A.slice_ptr, sp.data(), sp.size() * sizeof(int64_t), hipMemcpyHostToDevice, c->stream);
+    A.ncode = (int32_t)tab.size();
+    e = hipMemcpyAsync(A.slice_ptr, sp.data(), sp.size() * sizeof(int64_t), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(flag, 0, sizeof(int), c->stream);
+    if (e == hipSuccess && coded) e = hipMemsetAsync(A.code, 0, std::max<int64_t>(cbytes, 4), c->stream);
+    if (e == hipSuccess && coded)
+        e = hipMemcpyAsync(A.ctab, tab.data(), tab.size() * sizeof(int32_t), hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess && A.nslices > 0) {
         const int64_t threads = (int64_t)A.nslices * kSlice;
         hipLaunchKernelGGL((k_csr_to_sell<kGroupF64, double>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0,
                            c->stream, A.rowindex, A.colindex, A.values, A.slice_ptr, A.scol64, A.sval64, (int64_t)n,
-                           A.nslices, nullptr);
+                           A.nslices, flag, A.code, A.ctab, A.ncode);
         e = hipGetLastError();
     }
+    int bad = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&bad, flag, sizeof(int), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    if (e != hipSuccess) {
+    hipFree(flag);
+    if (e != hipSuccess || bad) {
         free_matrix(c);
-        set_error("matrix setup: %s", hipGetErrorString(e));
+        if (e != hipSuccess) {
+            set_error("matrix setup: %s", hipGetErrorString(e));
+            return HPCCG_ERR_HIP;
+        }
+        set_error("matrix setup: a column offset is missing from the code table");
         return HPCCG_ERR_HIP;
     }
     return 0;
@@ -993,6 +1082,18 @@ int hpccg_matrix_info(hpccg_ctx *c, int32_t *nrow, int64_t *nnz, int32_t *max_ro
     if (nrow) *nrow = c->A.nrow;
     if (nnz) *nnz = c->A.nnz;
     if (max_row_nnz) *max_row_nnz = c->A.max_row_nnz;
+    return 0;
+}
+
+int hpccg_matrix_format(hpccg_ctx *c, int32_t *ncodes, int64_t *copy_bytes)
+{
+    HPCCG_CHECK_ARG(c, "null context");
+    const Matrix &A = c->A;
+    if (ncodes) *ncodes = A.ncode;
+    if (copy_bytes)
+        *copy_bytes = A.slots * (int64_t)sizeof(double) +
+                      (A.code ? code_bytes(A.slots, A.nslices) : A.slots * (int64_t)sizeof(int32_t)) +
+                      (A.nrow ? (A.nslices + 1) * (int64_t)sizeof(int64_t) : 0);
     return 0;
 }
 

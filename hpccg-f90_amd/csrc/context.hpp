@@ -93,7 +93,10 @@ struct Matrix {
     int32_t nslices = 0;          // ceil(nrow / 64)
     int64_t slots = 0;            // slots of one copy (>= nnz; padding of short rows)
     int64_t *slice_ptr = nullptr; // nslices+1 slot offsets, shared by both copies
-    int32_t *scol64 = nullptr;    // FP64 copy: 0-based columns, 2 entries per lane group
+    uint8_t *code = nullptr;      // column codes shared by both copies (null: int32 columns)
+    int32_t *ctab = nullptr;      // their sorted offset table (kMaxCodes entries)
+    int32_t ncode = 0;            // offsets in the table
+    int32_t *scol64 = nullptr;    // FP64 copy: 0-based columns, 2 entries per lane group (uncoded only)
     double *sval64 = nullptr;
     int32_t *scol16 = nullptr;    // FP16 copy (mixed / learned paths, built on first use)
     __half *sval16 = nullptr;
@@ -102,11 +105,11 @@ struct Matrix {
 
 inline SellRef<double> sell64(const Matrix &A)
 {
-    return SellRef<double>{A.slice_ptr, A.scol64, A.sval64, A.rowindex, A.nrow, A.nslices};
+    return SellRef<double>{A.slice_ptr, A.scol64, A.sval64, A.rowindex, A.nrow, A.nslices, A.code, A.ctab, A.ncode};
 }
 inline SellRef<__half> sell16(const Matrix &A)
 {
-    return SellRef<__half>{A.slice_ptr, A.scol16, A.sval16, A.rowindex, A.nrow, A.nslices};
+    return SellRef<__half>{A.slice_ptr, A.scol16, A.sval16, A.rowindex, A.nrow, A.nslices, A.code, A.ctab, A.ncode};
 }
 // workgroups of a one-wave-per-slice launch (at least one, so reductions always run)
 inline int sell_grid(const Matrix &A, int waves = kSellWaves) { return A.nslices > 0 ? sell_blocks(A.nslices, waves) : 1; }

@@ -208,11 +208,12 @@ __global__ __launch_bounds__(kFinishThreads) void km_xr_finish(const double *__r
 
 // ---- host ------------------------------------------------------------------------------------
 // FP16 sliced copy of the matrix (built on first use): every value must be an FP16 number.
+// A coded matrix shares the FP64 copy's column codes; otherwise the copy gets int32 columns.
 int ensure_sell16(hpccg_ctx *c)
 {
     Matrix &A = c->A;
     if (A.sval16) return 0;
-    if (hipMalloc(&A.scol16, std::max<int64_t>(A.slots, 1) * sizeof(int32_t)) != hipSuccess ||
+    if ((!A.code && hipMalloc(&A.scol16, std::max<int64_t>(A.slots, 1) * sizeof(int32_t)) != hipSuccess) ||
         hipMalloc(&A.sval16, std::max<int64_t>(A.slots, 1) * sizeof(__half)) != hipSuccess) {
         hipFree(A.scol16);
         hipFree(A.sval16);
@@ -229,7 +230,7 @@ int ensure_sell16(hpccg_ctx *c)
         const int64_t threads = (int64_t)A.nslices * kSlice;
         hipLaunchKernelGGL((k_csr_to_sell<kGroupF16, __half>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0,
                            c->stream, A.rowindex, A.colindex, A.values, A.slice_ptr, A.scol16, A.sval16,
-                           (int64_t)A.nrow, A.nslices, flag);
+                           (int64_t)A.nrow, A.nslices, flag, nullptr, nullptr, 0);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(&bad, flag, sizeof(int), hipMemcpyDeviceToHost, c->stream);

@@ -13,7 +13,15 @@
 //     entries (fully coalesced) and each lane gets G consecutive entries of its own row;
 //   * columns are stored 0-based; slots past a row's end hold value 0 and a valid column;
 //   * slot offsets are multiples of 64, so bit 0 of slice_ptr[s] is free: it flags a "full"
-//     slice (64 rows, all exactly K_s long), whose row lengths need no rowindex reads.
+//     slice (64 rows, all exactly K_s long), whose row lengths need no rowindex reads;
+//   * column codes: when the matrix has at most kMaxCodes distinct offsets col - row (the
+//     27-point stencil has 27), columns are not stored as int32 but as one byte per slot, an
+//     index into the sorted offset table (column = row + table[code]).  The codes of slice s
+//     start at byte slice_base(slice_ptr[s]) + kCodePad * s; entry j of lane l is byte
+//     ((j / 4) * 64 + l) * 4 + j % 4 there, so one 4-byte load per lane fetches four codes.
+//     That cuts the index stream from 4 to 1 byte per slot (FP64 slot 12 -> 9 bytes, FP16
+//     6 -> 3) and serves the FP64 and FP16 copies alike.  A matrix with more distinct
+//     offsets keeps int32 columns.
 //
 // Each lane sums its own row in ascending entry order from 0.0 and skips the padding slots
 // (the row length comes from rowindex), so y = A x is bitwise the reference's row loop
@@ -143,14 +151,80 @@ __device__ __forceinline__ double sell_rowsum(const int64_t *__restrict__ slice_
     return sum;
 }
 
-// A sliced-ELL copy as kernels take it (by value).
+// ---- column codes ----------------------------------------------------------------------------
+constexpr int kMaxCodes = 256;              // distinct offsets col - row a coded copy can hold
+constexpr int64_t kCodePad = 3 * kSlice;    // code bytes per slice beyond its slots (K rounded up to 4)
+constexpr int kOffsetSlots = 4096;          // hash set that collects the offsets
+constexpr int32_t kNoOffset = INT32_MIN;    // empty slot (|col - row| < 2^31 - 1)
+__host__ __device__ __forceinline__ int64_t code_base(int64_t slot_base, int s) { return slot_base + kCodePad * s; }
+__host__ __device__ __forceinline__ int64_t code_bytes(int64_t slots, int nslices) { return slots + kCodePad * nslices; }
+__host__ __device__ __forceinline__ int64_t code_byte(int j, int lane) { return ((int64_t)(j >> 2) * kSlice + lane) * 4 + (j & 3); }
+
+// index of `off` in the sorted table tab[0..n) (n >= 1); tab[result] != off if it is absent
+__device__ __forceinline__ int column_code(const int32_t *__restrict__ tab, int n, int32_t off)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (tab[mid] < off) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Row sum as sell_rowsum, over a coded copy: the column of a slot is row + tab[code], with
+// tab the offset table in LDS.  Rows with no entries gather x[0] for their (skipped) padding.
+template <int G, typename VT, typename XT>
+__device__ __forceinline__ double sell_rowsum_coded(const int64_t *__restrict__ slice_ptr,
+                                                    const uint8_t *__restrict__ code, const int32_t *tab,
+                                                    const VT *__restrict__ sval, const int32_t *__restrict__ rowptr,
+                                                    const XT *__restrict__ x, int s, int lane, int row, bool valid)
+{
+    const int64_t sp0 = slice_ptr[s], base = slice_base(sp0);
+    const int K = (int)((slice_base(slice_ptr[s + 1]) - base) / kSlice);
+    const int len = !valid ? 0 : (sp0 & kSliceFull) ? K : rowptr[row + 1] - rowptr[row];
+    const uint32_t *cw = reinterpret_cast<const uint32_t *>(code + code_base(base, s)) + lane;
+    double sum = 0.0;
+    if (K == kStencilK) {
+        constexpr int NW = (kStencilK + 3) / 4, NG = kStencilK / G, KG = NG * G;
+        uint32_t w[NW];
+#pragma unroll
+        for (int g = 0; g < NW; ++g) w[g] = __builtin_nontemporal_load(cw + g * kSlice);
+        VT v[kStencilK];
+#pragma unroll
+        for (int g = 0; g < NG; ++g) ld_group<G>(sval + base + ((int64_t)g * kSlice + lane) * G, v + g * G);
+#pragma unroll
+        for (int j = KG; j < kStencilK; ++j) v[j] = ld_stream(sval + base + (int64_t)j * kSlice + lane);
+#pragma unroll
+        for (int j = 0; j < kStencilK; ++j) {
+            const int c = len > 0 ? row + tab[(w[j >> 2] >> (8 * (j & 3))) & 0xff] : 0;
+            const double prod = to_f64(v[j]) * to_f64(x[c]);
+            sum = (j < len) ? sum + prod : sum;
+        }
+    } else {
+        uint32_t wj = 0;
+        for (int j = 0; j < K; ++j) {
+            if ((j & 3) == 0) wj = __builtin_nontemporal_load(cw + (j >> 2) * kSlice);
+            const int c = len > 0 ? row + tab[(wj >> (8 * (j & 3))) & 0xff] : 0;
+            const double prod = to_f64(sval[sell_slot<G>(base, K, j, lane)]) * to_f64(x[c]);
+            sum = (j < len) ? sum + prod : sum;
+        }
+    }
+    return sum;
+}
+
+// A sliced-ELL copy as kernels take it (by value): int32 columns (col) or, when code is set,
+// column codes with their offset table (ntab entries).
 template <typename VT>
 struct SellRef {
     const int64_t *slice_ptr;  // nslices+1 slot offsets
-    const int32_t *col;        // 0-based columns
+    const int32_t *col;        // 0-based columns (null in a coded copy)
     const VT *val;
     const int32_t *rowptr;     // the 1-based CSR row pointer (row lengths)
     int nrow, nslices;
+    const uint8_t *code;       // column codes, or null
+    const int32_t *tab;        // sorted offsets col - row
+    int ntab;
 };
 
 // One wave per slice, WAVES slices per workgroup: the FP64 row sum of this thread's row (0 past
@@ -175,48 +249,67 @@ template <int G, typename VT, typename XT, int WAVES = kSellWaves, bool XRUN = f
 __device__ __forceinline__ double sell_rowsum_of_thread(const SellRef<VT> &A, const XT *__restrict__ x, int64_t &row,
                                                         bool &valid)
 {
+    // the offset table of a coded copy goes to LDS first (every thread of the workgroup gets here)
+    __shared__ int32_t tab[kMaxCodes];
+    if (A.code) {
+        for (int i = threadIdx.x; i < A.ntab; i += blockDim.x) tab[i] = A.tab[i];
+        __syncthreads();
+    }
     const int s = (XRUN ? sell_block() : (int)blockIdx.x) * WAVES + __builtin_amdgcn_readfirstlane(threadIdx.x / kSlice);
     const int lane = threadIdx.x % kSlice;
     row = (int64_t)s * kSlice + lane;
     valid = s < A.nslices && row < A.nrow;
     if (s >= A.nslices) return 0.0;
+    if (A.code) return sell_rowsum_coded<G, VT, XT>(A.slice_ptr, A.code, tab, A.val, A.rowptr, x, s, lane, (int)row, valid);
     return sell_rowsum<G, VT, XT>(A.slice_ptr, A.col, A.val, A.rowptr, x, s, lane, row, valid);
 }
 
 // CSR -> sliced ELL for the `nslices` slices of rows [0, nrow) of a 1-based CSR (rowptr,
-// col, val); the sliced copy stores 0-based columns.  One thread per slice row.  A VT
-// narrower than double raises *inexact for every value that does not convert exactly.
+// col, val).  One thread per slice row.  Columns go to scol (0-based) when it is set and as
+// codes into `codes` (offset table tab[0..ntab), sorted) when that is set.  A VT narrower
+// than double sets bit 0 of *flags for every value that does not convert exactly; an offset
+// missing from tab sets bit 1.
 template <int G, typename VT, typename VS = double>
 __global__ void k_csr_to_sell(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
                               const VS *__restrict__ val, const int64_t *__restrict__ slice_ptr,
                               int32_t *__restrict__ scol, VT *__restrict__ sval, int64_t nrow, int nslices,
-                              int *inexact)
+                              int *flags, uint8_t *__restrict__ codes, const int32_t *__restrict__ tab, int ntab)
 {
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (t >= (int64_t)nslices * kSlice) return;
     const int s = (int)(t / kSlice), lane = (int)(t % kSlice);
     const int64_t base = slice_base(slice_ptr[s]);
     const int K = (int)((slice_base(slice_ptr[s + 1]) - base) / kSlice);
-    int len = 0, padc = 0, bad = 0;
+    uint8_t *cb = codes ? codes + code_base(base, s) : nullptr;
+    int len = 0, padc = 0, bad = 0, padcode = 0;
+    auto put_code = [&](int j, int c) {
+        const int32_t off = c - (int32_t)t;
+        const int code = column_code(tab, ntab, off);
+        if (tab[code] != off) bad |= 2;
+        cb[code_byte(j, lane)] = (uint8_t)code;
+        return code;
+    };
     if (t < nrow) {
         const int64_t k0 = rowptr[t] - 1;
         len = rowptr[t + 1] - rowptr[t];
         for (int j = 0; j < len; ++j) {
             const int64_t q = sell_slot<G>(base, K, j, lane);
             const double v = to_f64(val[k0 + j]);
-            scol[q] = col[k0 + j] - 1;
+            if (scol) scol[q] = col[k0 + j] - 1;
+            if (cb) padcode = put_code(j, col[k0 + j] - 1);
             const VT w = (VT)v;
             sval[q] = w;
-            if (to_f64(w) != v) bad = 1;
+            if (to_f64(w) != v) bad |= 1;
         }
         padc = len > 0 ? col[k0 + len - 1] - 1 : 0;
     }
     for (int j = len; j < K; ++j) {
         const int64_t q = sell_slot<G>(base, K, j, lane);
-        scol[q] = padc;
+        if (scol) scol[q] = padc;
+        if (cb) cb[code_byte(j, lane)] = (uint8_t)(len > 0 ? padcode : 0);
         sval[q] = (VT)0.0;
     }
-    if (bad && inexact) atomicOr(inexact, 1);
+    if (bad && flags) atomicOr(flags, bad);
 }
 
 // Host: slot offsets (with the full-slice flags) of the slices of rows [0, nrow) of a 1-based

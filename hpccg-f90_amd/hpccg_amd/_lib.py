@@ -39,6 +39,7 @@ _SIGS = {
     "hpccg_matrix_upload": (ctypes.c_int, [P, I32, I32, P, P, P]),
     "hpccg_generate_matrix": (ctypes.c_int, [P, I32, I32, I32, P, P, P, P, P, P, PI64]),
     "hpccg_matrix_info": (ctypes.c_int, [P, PI32, PI64, PI32]),
+    "hpccg_matrix_format": (ctypes.c_int, [P, PI32, PI64]),
     "hpccg_spmv": (ctypes.c_int, [P, P, P]),
     "hpccg_ddot": (ctypes.c_int, [P, I32, P, P, PD]),
     "hpccg_ddot1": (ctypes.c_int, [P, I32, P, PD]),
@@ -250,6 +251,13 @@ class Context:
         n, nnz, mr = ctypes.c_int32(), ctypes.c_int64(), ctypes.c_int32()
         _check(lib().hpccg_matrix_info(self.h, ctypes.byref(n), ctypes.byref(nnz), ctypes.byref(mr)))
         return {"nrow": n.value, "nnz": nnz.value, "max_row_nnz": mr.value}
+
+    def matrix_format(self):
+        """Device format of the uploaded matrix: `ncodes` offsets in the column-code table (0:
+        int32 columns) and `copy_bytes` of the FP64 sliced copy."""
+        nc, cb = ctypes.c_int32(), ctypes.c_int64()
+        _check(lib().hpccg_matrix_format(self.h, ctypes.byref(nc), ctypes.byref(cb)))
+        return {"ncodes": nc.value, "copy_bytes": cb.value}
 
     # ---- host-array drop-ins
     def spmv(self, x):

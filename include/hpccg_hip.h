@@ -73,6 +73,12 @@ int hpccg_generate_matrix(hpccg_ctx *ctx, int32_t nx, int32_t ny, int32_t nz, in
                           int32_t *colindex, double *values, double *x, double *b, double *xexact,
                           int64_t *nnz_out);
 int hpccg_matrix_info(hpccg_ctx *ctx, int32_t *nrow, int64_t *nnz, int32_t *max_row_nnz);
+/* Device format of the uploaded matrix (no reference counterpart: an inspection call).
+ * *ncodes: size of the column-offset table when the sliced copies store one-byte column codes
+ * (column = row + table[code]; used when the matrix has at most 256 distinct offsets col - row,
+ * 27 for the 27-point stencil), 0 when they store int32 columns.  *copy_bytes: device bytes of
+ * the FP64 sliced copy (values, columns or codes, slice pointers). */
+int hpccg_matrix_format(hpccg_ctx *ctx, int32_t *ncodes, int64_t *copy_bytes);
 
 /* ---- kernels, host arrays (drop-ins for the reference procedures) ---------------------- */
 /* spmv(A, x, y)             SparseMatrix.F90:213-232 */

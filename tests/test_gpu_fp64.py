@@ -140,6 +140,63 @@ def test_spmv_generic_csr_long_rows(ctx):
     assert ctx.matrix_info()["max_row_nnz"] == lens.max()
 
 
+def test_spmv_generic_csr_keeps_int32_columns(ctx):
+    """Thousands of distinct offsets col - row: the sliced copy keeps int32 columns."""
+    rng = np.random.default_rng(5)
+    n = 2000
+    lens = rng.integers(1, 40, n)
+    rowindex = np.concatenate([[1], 1 + np.cumsum(lens)]).astype(np.int32)
+    cols = np.concatenate([np.sort(rng.choice(n, size=l, replace=False)) + 1 for l in lens]).astype(np.int32)
+    vals = rng.standard_normal(len(cols))
+    ctx.upload(rowindex, cols, vals)
+    assert ctx.matrix_format()["ncodes"] == 0
+    A = type("P", (), {})()  # minimal oracle.Problem stand-in for oracle.spmv
+    A.n, A.rowindex, A.colindex, A.values = n, rowindex, cols, vals
+    x = rng.standard_normal(n)
+    assert np.array_equal(ctx.spmv(x), oracle.spmv(A, x))
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 1000, 4099])
+def test_spmv_column_codes_banded_with_empty_rows(ctx, n):
+    """At most 256 distinct offsets: one-byte column codes.  A banded CSR with 40 offsets (so
+    slices are not 27 wide), rows of 0..40 entries, empty rows and a ragged last slice."""
+    rng = np.random.default_rng(n)
+    offsets = np.sort(rng.choice(np.arange(-300, 301), size=40, replace=False))
+    rows = []
+    for i in range(n):
+        c = i + offsets
+        c = c[(c >= 0) & (c < n)]
+        keep = rng.random(len(c)) < (0.0 if i % 11 == 3 else 0.8)
+        rows.append(c[keep])
+    lens = np.array([len(r) for r in rows])
+    rowindex = np.concatenate([[1], 1 + np.cumsum(lens)]).astype(np.int32)
+    cols = (np.concatenate(rows) + 1).astype(np.int32) if lens.sum() else np.zeros(0, np.int32)
+    vals = rng.standard_normal(len(cols))
+    if len(cols) == 0:
+        pytest.skip("no entries")
+    ctx.upload(rowindex, cols, vals)
+    fmt = ctx.matrix_format()
+    assert 1 <= fmt["ncodes"] <= 40
+    A = type("P", (), {})()
+    A.n, A.rowindex, A.colindex, A.values = n, rowindex, cols, vals
+    x = rng.standard_normal(n)
+    assert np.array_equal(ctx.spmv(x), oracle.spmv(A, x))
+
+
+def test_generated_matrix_uses_27_column_codes(ctx):
+    """The 27-point matrix is stored with 27 offset codes: 9 bytes per slot instead of 12."""
+    nx, ny, nz = 20, 30, 10
+    out = ctx.generate(nx, ny, nz, want_host=True)
+    fmt = ctx.matrix_format()
+    assert fmt["ncodes"] == 27
+    n = nx * ny * nz
+    ns = (n + 63) // 64
+    lens = np.zeros(ns * 64, np.int64)
+    lens[:n] = np.diff(out["rowindex"])
+    slots = 64 * int(lens.reshape(ns, 64).max(axis=1).sum())  # slice width = its longest row
+    assert fmt["copy_bytes"] == slots * (8 + 1) + 192 * ns + (ns + 1) * 8
+
+
 def test_upload_rejects_bad_csr(ctx):
     with pytest.raises(hpccg_amd.HpccgError):
         ctx.upload(np.array([0, 1], np.int32), np.array([1], np.int32), np.array([1.0]))  # 0-based
