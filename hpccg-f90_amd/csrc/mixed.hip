@@ -137,7 +137,7 @@ __global__ __launch_bounds__(kSellThreads) void km_spmv(const SellRef<__half> A,
     if (st->k >= st->stop_at) return;
     int64_t row;
     bool valid;
-    const double s = sell_rowsum_of_thread<kGroupF16>(A, p, row, valid);
+    const double s = sell_rowsum_of_thread<kGroupF16, __half, float, kSellWaves, true>(A, p, row, valid);
     double dot = 0.0;
     if (valid) {
         const float a = (float)s;

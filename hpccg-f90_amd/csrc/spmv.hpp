@@ -243,8 +243,9 @@ __device__ __forceinline__ int sell_block()
     return (b / span) * span + (b % kNumXcd) * kXcdRun + (b / kNumXcd) % kXcdRun;
 }
 
-// XRUN: number the workgroups with sell_block() (the FP64 kernels; the FP16 ones measured
-// slower with it: mixed CG 49.9 -> 51.0 ms, flexible CG 26.7 -> 27.3 ms at 256^3)
+// XRUN: number the workgroups with sell_block().  Used by the FP64 SpMV and the mixed-precision
+// FP16 SpMV (coded copy at 256^3: mixed CG 36.0 -> 35.6 ms); the flexible-CG SpMV, which reads
+// the preconditioner's output just written, keeps hardware order (22.2 -> 22.4 ms with it).
 template <int G, typename VT, typename XT, int WAVES = kSellWaves, bool XRUN = false>
 __device__ __forceinline__ double sell_rowsum_of_thread(const SellRef<VT> &A, const XT *__restrict__ x, int64_t &row,
                                                         bool &valid)
