@@ -6,7 +6,8 @@
 //
 // Layout: the systems' CSRs are concatenated into one block-diagonal CSR (global 1-based
 // row pointers and columns), vectors are concatenated by row offset, and the sliced-ELL copy
-// starts every system on a new 64-row slice.  Every kernel covers all systems at once and is
+// starts every system on a new 64-row slice.  The copy stores one-byte column codes, each an
+// index into its system's table of stencil offsets col - row.  Every kernel covers all systems at once and is
 // slice-parallel: a wave owns `rows per wave` / 64 consecutive slices of the whole batch and
 // writes one dot partial per slice; a per-system finish kernel (one workgroup per system) sums
 // its slices' partials in a fixed order and updates that system's scalars.  A converged
@@ -51,7 +52,8 @@ struct Batch {
     int64_t *row_off = nullptr;         // device copy of row_off_h
     // sliced-ELL copy (spmv.hpp), slices never straddle two systems
     int64_t *slice_ptr = nullptr;       // [total slices + 1] slot offsets
-    int32_t *scol = nullptr;            // 0-based global columns
+    uint8_t *code = nullptr;            // column codes (spmv.hpp), offsets relative to each system
+    int32_t *tabs = nullptr;            // [S][kBatchTab] each system's sorted offset table
     __half *sval = nullptr;
     int *slice_off = nullptr;           // [S+1] first slice of each system
     int *slice_sys = nullptr;           // [NS] system of each slice
@@ -110,10 +112,13 @@ __global__ void kb_generate(int nx, int ny, int nz, int64_t row_off, int64_t nnz
 __global__ void kb_set_last(int32_t *rowindex, int64_t E, int64_t nnz) { rowindex[E] = (int32_t)(nnz + 1); }
 
 // ---- slice helpers -----------------------------------------------------------------------------
-// The batch's sliced copy as the kernels take it.
+// The batch's sliced copy as the kernels take it.  Columns are one-byte codes into the slice's
+// system's table of its (at most 27) stencil offsets col - row.
+constexpr int kBatchTab = 32;  // table entries per system (27 used, the rest repeat the last)
 struct BatchSell {
     const int64_t *slice_ptr;
-    const int32_t *col;
+    const uint8_t *code;
+    const int32_t *tabs;
     const __half *val;
     const int32_t *rowptr;  // global 1-based row pointer
     const int *slice_off;   // first slice of each system
@@ -124,7 +129,7 @@ struct BatchSell {
 
 static inline BatchSell batch_sell(const Batch *B)
 {
-    return BatchSell{B->slice_ptr, B->scol, B->sval, B->rowindex, B->slice_off, B->slice_sys, B->row_off, B->NS};
+    return BatchSell{B->slice_ptr, B->code, B->tabs, B->sval, B->rowindex, B->slice_off, B->slice_sys, B->row_off, B->NS};
 }
 
 // Where slice gs lies: its system, the lane's global row and whether that row exists.
@@ -155,6 +160,19 @@ __device__ __forceinline__ void wave_slices(int NS, F f)
         if (g0 + j < NS) f(g0 + j);
 }
 
+// Row sum (FP64, the reference's entry order) of this lane's row of slice gs.  The slice's
+// system table is held one entry per lane (lane i: entry i % kBatchTab) and each code is looked
+// up with a cross-lane read; all 64 lanes of the wave take part.
+template <typename XT>
+__device__ __forceinline__ double batch_rowsum(const BatchSell &A, const SlicePos &q, const XT *__restrict__ x, int gs,
+                                               int lane)
+{
+    const int tv = A.tabs[kBatchTab * q.s + lane % kBatchTab];
+    return sell_rowsum_coded<kGroupF16, __half, XT>(
+        A.slice_ptr, A.code, [&](int c) { return __builtin_amdgcn_ds_bpermute(c << 2, tv); }, A.val, A.rowptr, x, gs,
+        lane, (int)q.gr, q.valid);
+}
+
 // Sum of the slice partials [slice_off[s], slice_off[s+1]) (stride `stride`, offset `o`) in
 // a fixed order; every thread of the 256-thread workgroup gets it.
 __device__ __forceinline__ double system_sum(const double *__restrict__ part, const int *__restrict__ slice_off, int s,
@@ -179,8 +197,7 @@ __global__ __launch_bounds__(W * 64) void kb_true(const BatchSell A, const doubl
         const SlicePos q = slice_pos(A, gs, lane);
         const BatchState *S = st + q.s;
         if (!PREP && (!S->active || !S->need_true)) return;
-        const double ax =
-            sell_rowsum<kGroupF16, __half, double>(A.slice_ptr, A.col, A.val, A.rowptr, x, gs, lane, q.gr, q.valid);
+        const double ax = batch_rowsum(A, q, x, gs, lane);
         double rr = 0.0, bb = 0.0;
         if (q.valid) {
             const double bi = b[q.gr];
@@ -265,8 +282,7 @@ __global__ __launch_bounds__(W * 64) void kb_spmv(const BatchSell A, const float
     wave_slices<W, SPW>(A.NS, [&](int gs) {
         const SlicePos q = slice_pos(A, gs, lane);
         if (!st[q.s].active) return;
-        const double sum =
-            sell_rowsum<kGroupF16, __half, float>(A.slice_ptr, A.col, A.val, A.rowptr, p, gs, lane, q.gr, q.valid);
+        const double sum = batch_rowsum(A, q, p, gs, lane);
         double dot = 0.0;
         if (q.valid) {
             const float a = (float)sum;
@@ -352,7 +368,8 @@ void free_batched(hpccg_ctx *c)
     hipFree(B->values);
     hipFree(B->row_off);
     hipFree(B->slice_ptr);
-    hipFree(B->scol);
+    hipFree(B->code);
+    hipFree(B->tabs);
     hipFree(B->sval);
     hipFree(B->slice_off);
     hipFree(B->r);
@@ -512,7 +529,8 @@ int hpccg_batch_setup(hpccg_ctx *c, int32_t count, const int32_t *dims)
                        sp.data() + soff[s]);
     B->slots = sp[soff[count]];
     if (hipMalloc(&B->slice_ptr, sp.size() * sizeof(int64_t)) != hipSuccess ||
-        hipMalloc(&B->scol, std::max<int64_t>(B->slots, 1) * sizeof(int32_t)) != hipSuccess ||
+        hipMalloc(&B->code, std::max<int64_t>(code_bytes(B->slots, soff[count]), 4)) != hipSuccess ||
+        hipMalloc(&B->tabs, (size_t)count * kBatchTab * sizeof(int32_t)) != hipSuccess ||
         hipMalloc(&B->sval, std::max<int64_t>(B->slots, 1) * sizeof(__half)) != hipSuccess ||
         hipMalloc(&B->slice_off, (count + 1) * sizeof(int)) != hipSuccess) {
         free_batched(c);
@@ -533,16 +551,46 @@ int hpccg_batch_setup(hpccg_ctx *c, int32_t count, const int32_t *dims)
     HPCCG_HIP_TRY(hipMemcpy(B->slice_sys, ssys.data(), ssys.size() * sizeof(int), hipMemcpyHostToDevice));
     HPCCG_HIP_TRY(hipMemcpy(B->slice_ptr, sp.data(), sp.size() * sizeof(int64_t), hipMemcpyHostToDevice));
     HPCCG_HIP_TRY(hipMemcpy(B->slice_off, soff.data(), (count + 1) * sizeof(int), hipMemcpyHostToDevice));
+    // each system's table: the distinct stencil offsets dz*nx*ny + dy*nx + dx, sorted
+    std::vector<int32_t> tabs((size_t)count * kBatchTab);
+    std::vector<int> ntab(count);
     for (int s = 0; s < count; ++s) {
+        const int32_t nx = dims[3 * s], ny = dims[3 * s + 1];
+        std::vector<int32_t> t;
+        for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) t.push_back(dz * nx * ny + dy * nx + dx);
+        std::sort(t.begin(), t.end());
+        t.erase(std::unique(t.begin(), t.end()), t.end());
+        ntab[s] = (int)t.size();
+        for (int i = 0; i < kBatchTab; ++i) tabs[(size_t)s * kBatchTab + i] = t[std::min(i, ntab[s] - 1)];
+    }
+    int *flag = nullptr;
+    HPCCG_HIP_TRY(hipMalloc(&flag, sizeof(int)));
+    int bad = 0;
+    hipError_t e = hipMemcpyAsync(B->tabs, tabs.data(), tabs.size() * sizeof(int32_t), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(flag, 0, sizeof(int), c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(B->code, 0, std::max<int64_t>(code_bytes(B->slots, B->NS), 4), c->stream);
+    for (int s = 0; s < count && e == hipSuccess; ++s) {
         const int ns = soff[s + 1] - soff[s];
         const int64_t threads = (int64_t)ns * kSlice;
+        // codes of system s: offsets relative to its own rows (col_shift = its first row)
         hipLaunchKernelGGL((k_csr_to_sell<kGroupF16, __half, __half>), dim3((unsigned)((threads + 255) / 256)),
                            dim3(256), 0, c->stream, B->rowindex + B->row_off_h[s], B->colindex, B->values,
-                           B->slice_ptr + soff[s], B->scol, B->sval, B->row_off_h[s + 1] - B->row_off_h[s], ns,
-                           nullptr, nullptr, nullptr, 0);
+                           B->slice_ptr + soff[s], (int32_t *)nullptr, B->sval, B->row_off_h[s + 1] - B->row_off_h[s],
+                           ns, flag, B->code + kCodePad * soff[s], B->tabs + (size_t)s * kBatchTab, ntab[s],
+                           (int32_t)B->row_off_h[s]);
+        e = hipGetLastError();
     }
-    HPCCG_HIP_TRY(hipGetLastError());
-    HPCCG_HIP_TRY(hipStreamSynchronize(c->stream));
+    if (e == hipSuccess) e = hipMemcpyAsync(&bad, flag, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipFree(flag);
+    if (e != hipSuccess || bad) {
+        free_batched(c);
+        set_error(e != hipSuccess ? "batched: sliced copy: %s" : "batched: a column offset is missing from its table%s",
+                  e != hipSuccess ? hipGetErrorString(e) : "");
+        return HPCCG_ERR_HIP;
+    }
     return 0;
 }
 

@@ -172,11 +172,12 @@ __device__ __forceinline__ int column_code(const int32_t *__restrict__ tab, int 
     return lo;
 }
 
-// Row sum as sell_rowsum, over a coded copy: the column of a slot is row + tab[code], with
-// tab the offset table in LDS.  Rows with no entries gather x[0] for their (skipped) padding.
-template <int G, typename VT, typename XT>
+// Row sum as sell_rowsum, over a coded copy: the column of a slot is row + lookup(code), the
+// offset table entry (from LDS, or from another lane's register).  Every lane of the wave runs
+// the lookups.  Rows with no entries gather x[0] for their (skipped) padding.
+template <int G, typename VT, typename XT, typename Lookup>
 __device__ __forceinline__ double sell_rowsum_coded(const int64_t *__restrict__ slice_ptr,
-                                                    const uint8_t *__restrict__ code, const int32_t *tab,
+                                                    const uint8_t *__restrict__ code, Lookup lookup,
                                                     const VT *__restrict__ sval, const int32_t *__restrict__ rowptr,
                                                     const XT *__restrict__ x, int s, int lane, int row, bool valid)
 {
@@ -197,7 +198,8 @@ __device__ __forceinline__ double sell_rowsum_coded(const int64_t *__restrict__ 
         for (int j = KG; j < kStencilK; ++j) v[j] = ld_stream(sval + base + (int64_t)j * kSlice + lane);
 #pragma unroll
         for (int j = 0; j < kStencilK; ++j) {
-            const int c = len > 0 ? row + tab[(w[j >> 2] >> (8 * (j & 3))) & 0xff] : 0;
+            const int off = lookup((w[j >> 2] >> (8 * (j & 3))) & 0xff);
+            const int c = len > 0 ? row + off : 0;
             const double prod = to_f64(v[j]) * to_f64(x[c]);
             sum = (j < len) ? sum + prod : sum;
         }
@@ -205,7 +207,8 @@ __device__ __forceinline__ double sell_rowsum_coded(const int64_t *__restrict__ 
         uint32_t wj = 0;
         for (int j = 0; j < K; ++j) {
             if ((j & 3) == 0) wj = __builtin_nontemporal_load(cw + (j >> 2) * kSlice);
-            const int c = len > 0 ? row + tab[(wj >> (8 * (j & 3))) & 0xff] : 0;
+            const int off = lookup((wj >> (8 * (j & 3))) & 0xff);
+            const int c = len > 0 ? row + off : 0;
             const double prod = to_f64(sval[sell_slot<G>(base, K, j, lane)]) * to_f64(x[c]);
             sum = (j < len) ? sum + prod : sum;
         }
@@ -261,20 +264,25 @@ __device__ __forceinline__ double sell_rowsum_of_thread(const SellRef<VT> &A, co
     row = (int64_t)s * kSlice + lane;
     valid = s < A.nslices && row < A.nrow;
     if (s >= A.nslices) return 0.0;
-    if (A.code) return sell_rowsum_coded<G, VT, XT>(A.slice_ptr, A.code, tab, A.val, A.rowptr, x, s, lane, (int)row, valid);
+    if (A.code)
+        return sell_rowsum_coded<G, VT, XT>(A.slice_ptr, A.code, [&](int c) { return tab[c]; }, A.val, A.rowptr, x, s,
+                                            lane, (int)row, valid);
     return sell_rowsum<G, VT, XT>(A.slice_ptr, A.col, A.val, A.rowptr, x, s, lane, row, valid);
 }
 
 // CSR -> sliced ELL for the `nslices` slices of rows [0, nrow) of a 1-based CSR (rowptr,
 // col, val).  One thread per slice row.  Columns go to scol (0-based) when it is set and as
-// codes into `codes` (offset table tab[0..ntab), sorted) when that is set.  A VT narrower
+// codes into `codes` (offset table tab[0..ntab), sorted) when that is set; a code stands for
+// the offset (col - 1 - col_shift) - row, so that columns of a sub-block (rows and columns
+// shifted by col_shift in a larger matrix) are coded relative to that block.  A VT narrower
 // than double sets bit 0 of *flags for every value that does not convert exactly; an offset
 // missing from tab sets bit 1.
 template <int G, typename VT, typename VS = double>
 __global__ void k_csr_to_sell(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
                               const VS *__restrict__ val, const int64_t *__restrict__ slice_ptr,
                               int32_t *__restrict__ scol, VT *__restrict__ sval, int64_t nrow, int nslices,
-                              int *flags, uint8_t *__restrict__ codes, const int32_t *__restrict__ tab, int ntab)
+                              int *flags, uint8_t *__restrict__ codes, const int32_t *__restrict__ tab, int ntab,
+                              int32_t col_shift = 0)
 {
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (t >= (int64_t)nslices * kSlice) return;
@@ -284,7 +292,7 @@ __global__ void k_csr_to_sell(const int32_t *__restrict__ rowptr, const int32_t 
     uint8_t *cb = codes ? codes + code_base(base, s) : nullptr;
     int len = 0, padc = 0, bad = 0, padcode = 0;
     auto put_code = [&](int j, int c) {
-        const int32_t off = c - (int32_t)t;
+        const int32_t off = c - col_shift - (int32_t)t;
         const int code = column_code(tab, ntab, off);
         if (tab[code] != off) bad |= 2;
         cb[code_byte(j, lane)] = (uint8_t)code;
