@@ -582,6 +582,11 @@ int hpccg_batch_setup(hpccg_ctx *c, int32_t count, const int32_t *dims)
                            (int32_t)B->row_off_h[s]);
         e = hipGetLastError();
     }
+    if (e == hipSuccess && B->NS > 0) {
+        hipLaunchKernelGGL(k_mark_uniform<>, dim3((unsigned)(((int64_t)B->NS * kSlice + 255) / 256)), dim3(256), 0,
+                           c->stream, B->slice_ptr, B->code, B->NS);
+        e = hipGetLastError();
+    }
     if (e == hipSuccess) e = hipMemcpyAsync(&bad, flag, sizeof(int), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     hipFree(flag);

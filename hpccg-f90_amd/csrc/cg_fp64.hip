@@ -968,6 +968,9 @@ static int finish_matrix_setup(hpccg_ctx *c, const int32_t *rowindex_host)
         hipLaunchKernelGGL((k_csr_to_sell<kGroupF64, double>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0,
                            c->stream, A.rowindex, A.colindex, A.values, A.slice_ptr, A.scol64, A.sval64, (int64_t)n,
                            A.nslices, flag, A.code, A.ctab, A.ncode);
+        if (coded)
+            hipLaunchKernelGGL(k_mark_uniform<>, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, c->stream,
+                               A.slice_ptr, A.code, A.nslices);
         e = hipGetLastError();
     }
     int bad = 0;

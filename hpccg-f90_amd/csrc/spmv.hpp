@@ -107,6 +107,7 @@ __host__ __device__ __forceinline__ int64_t sell_slot(int64_t base, int K, int j
 }
 
 constexpr int64_t kSliceFull = 1;                       // flag bit of slice_ptr[s]
+constexpr int64_t kSliceUniform = 2;  // flag bit: coded full slice whose lanes all carry lane 0's codes
 __host__ __device__ __forceinline__ int64_t slice_base(int64_t sp) { return sp & ~(int64_t)(kSlice - 1); }
 
 // Row sum (FP64, ascending entry order from 0.0) of global row `row` held by lane `lane` of
@@ -160,6 +161,24 @@ __host__ __device__ __forceinline__ int64_t code_base(int64_t slot_base, int s) 
 __host__ __device__ __forceinline__ int64_t code_bytes(int64_t slots, int nslices) { return slots + kCodePad * nslices; }
 __host__ __device__ __forceinline__ int64_t code_byte(int j, int lane) { return ((int64_t)(j >> 2) * kSlice + lane) * 4 + (j & 3); }
 
+// Flags (kSliceUniform) each full slice of a coded copy whose 64 lanes all carry lane 0's
+// codes.  One wave per slice; only that wave writes slice_ptr[s], and the flag bits never
+// change the slot offsets other waves read from it.
+template <typename Unused = void>
+__global__ void k_mark_uniform(int64_t *slice_ptr, const uint8_t *__restrict__ code, int nslices)
+{
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int s = (int)(t / kSlice), lane = (int)(t % kSlice);
+    if (s >= nslices) return;
+    const int64_t sp0 = slice_ptr[s], base = slice_base(sp0);
+    if (!(sp0 & kSliceFull)) return;
+    const int K = (int)((slice_base(slice_ptr[s + 1]) - base) / kSlice);
+    const uint32_t *cw = reinterpret_cast<const uint32_t *>(code + code_base(base, s));
+    int same = 1;
+    for (int g = 0; g < (K + 3) / 4; ++g) same &= cw[g * kSlice + lane] == cw[g * kSlice];
+    if (__all(same) && lane == 0) slice_ptr[s] = sp0 | kSliceUniform;
+}
+
 // index of `off` in the sorted table tab[0..n) (n >= 1); tab[result] != off if it is absent
 __device__ __forceinline__ int column_code(const int32_t *__restrict__ tab, int n, int32_t off)
 {
@@ -184,7 +203,8 @@ __device__ __forceinline__ double sell_rowsum_coded(const int64_t *__restrict__ 
     const int64_t sp0 = slice_ptr[s], base = slice_base(sp0);
     const int K = (int)((slice_base(slice_ptr[s + 1]) - base) / kSlice);
     const int len = !valid ? 0 : (sp0 & kSliceFull) ? K : rowptr[row + 1] - rowptr[row];
-    const uint32_t *cw = reinterpret_cast<const uint32_t *>(code + code_base(base, s)) + lane;
+    // a uniform slice (interior rows of a stencil) reads lane 0's codes only: one address per load
+    const uint32_t *cw = reinterpret_cast<const uint32_t *>(code + code_base(base, s)) + ((sp0 & kSliceUniform) ? 0 : lane);
     double sum = 0.0;
     if (K == kStencilK) {
         constexpr int NW = (kStencilK + 3) / 4, NG = kStencilK / G, KG = NG * G;

@@ -183,6 +183,17 @@ def test_spmv_column_codes_banded_with_empty_rows(ctx, n):
     assert np.array_equal(ctx.spmv(x), oracle.spmv(A, x))
 
 
+@pytest.mark.parametrize("dims", [(130, 5, 4), (256, 4, 3), (64, 3, 3)])
+def test_spmv_uniform_code_slices_bitwise(ctx, dims):
+    """Grids whose x-lines hold whole 64-row slices of interior rows: those slices read one
+    shared code sequence.  The product must stay the reference's bit for bit."""
+    A = oracle.Problem(*dims)
+    ctx.upload(A.rowindex, A.colindex, A.values)
+    assert ctx.matrix_format()["ncodes"] == 27
+    x = np.random.default_rng(sum(dims)).standard_normal(A.n)
+    assert np.array_equal(ctx.spmv(x), oracle.spmv(A, x))
+
+
 def test_generated_matrix_uses_27_column_codes(ctx):
     """The 27-point matrix is stored with 27 offset codes: 9 bytes per slot instead of 12."""
     nx, ny, nz = 20, 30, 10
