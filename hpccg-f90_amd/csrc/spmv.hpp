@@ -40,8 +40,9 @@ namespace hpccg {
 constexpr int kSlice = 64;            // rows per slice == lanes per wave
 constexpr int kSellWaves = 4;         // slices per SpMV workgroup (one per wave)
 constexpr int kSellThreads = kSlice * kSellWaves;
-// true-residual kernels run only every few iterations and exit at once otherwise: 16 slices
-// per workgroup keep the cost of those empty launches low (16x fewer workgroups)
+// true-residual kernels (they run only every few iterations and exit at once otherwise): 4
+// slices per workgroup.  16 made the empty launches cheaper but a real one slower (256^3:
+// 0.75 ms against 0.66 ms), and a real one costs more than all empty ones of a solve.
 constexpr int kTrueWaves = 4;
 constexpr int kTrueThreads = kSlice * kTrueWaves;
 constexpr int kStencilK = 27;         // slice width of the 27-point stencil: fully unrolled path
