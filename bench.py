@@ -245,6 +245,16 @@ def main():
                                  fell_back=res["fell_back"])
         log(f"{name}: {results[name]}")
 
+    # the host-array drop-in (hpccg_cg, what the Fortran cg calls): b in and x out over PCIe
+    # around the same FP64 solve; reported beside `value`, never as it (one warm-up call first)
+    for _ in range(2):
+        t = time.perf_counter()
+        xh, nit_h, _, _, _ = ctx.cg(b_h, max_iter=2000, tolerance=REL_TOL * normb)
+        t = time.perf_counter() - t
+    host_arrays = {"s_wall": round(t, 6), "niters": nit_h, "xerr_max": float(np.max(np.abs(xh - xs_h))),
+                   "pcie_bytes": 16 * n}
+    log(f"fp64_cg through host arrays: {host_arrays}")
+
     # headline: the fastest method whose answer meets 1e-10 in the FP64 true residual
     ok = [k for k, v in results.items() if v["true_relres"] <= REL_TOL]
     headline = min(ok, key=lambda k: results[k]["s_wall"]) if ok else "fp64_cg"
@@ -308,6 +318,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "solvers": results,
+        "fp64_cg_host_arrays": host_arrays,
         "learned_training": train_info,
         "batched": batched,
     }
