@@ -33,7 +33,8 @@ def _batch(ctx, dims, seed):
 
 
 def test_batch_spmv_bitwise_per_system(ctx):
-    dims = np.array([[5, 4, 3], [16, 9, 7], [1, 1, 1], [33, 2, 5], [12, 12, 12]], np.int32)
+    # [130, 5, 4] and [64, 3, 3] hold uniform-code slices (whole 64-row runs of interior rows)
+    dims = np.array([[5, 4, 3], [16, 9, 7], [1, 1, 1], [33, 2, 5], [12, 12, 12], [130, 5, 4], [64, 3, 3]], np.int32)
     info = ctx.batch_setup(dims)
     off = info["row_off"]
     x32 = oracle.xstar_uniform(info["rows"], 5).astype(np.float32)
