@@ -21,14 +21,17 @@
 //     ((j / 4) * 64 + l) * 4 + j % 4 there, so one 4-byte load per lane fetches four codes.
 //     That cuts the index stream from 4 to 1 byte per slot (FP64 slot 12 -> 9 bytes, FP16
 //     6 -> 3) and serves the FP64 and FP16 copies alike.  A matrix with more distinct
-//     offsets keeps int32 columns.
+//     offsets keeps int32 columns.  Bit 1 of slice_ptr[s] flags a "uniform" coded slice (full,
+//     and every lane carries lane 0's codes: the interior rows of a stencil), whose lanes
+//     all read lane 0's code words, so the other lanes' code lines are never fetched.
 //
 // Each lane sums its own row in ascending entry order from 0.0 and skips the padding slots
 // (the row length comes from rowindex), so y = A x is bitwise the reference's row loop
 // (SparseMatrix.F90:225-229) for FP64 values -- and for FP16 values it is the FP64 row sum
 // of the exactly widened products.  Compared with a CSR kernel this layout needs no LDS
-// staging and no per-row scatter: the value and column streams are read with unit stride
-// and the x gathers of neighbouring lanes touch neighbouring rows' columns.
+// staging of the matrix and no per-row scatter: the value and column (or code) streams are
+// read with unit stride and the x gathers of neighbouring lanes touch neighbouring rows'
+// columns.
 #pragma once
 
 #include <hip/hip_fp16.h>
