@@ -449,14 +449,18 @@ __global__ __launch_bounds__(kTrueThreads) void kf_true(const SellRef<__half> A,
 {
     __shared__ double red[kTrueThreads / kWave];
     if (!kf_true_due<PREP>(st)) return;
-    int64_t row;
-    bool valid;
-    const double ax = sell_rowsum_of_thread<kGroupF16, __half, double, kTrueWaves>(A, x, row, valid);
     double dot = 0.0;
-    if (valid) {
-        const float ri = (float)(b[row] - ax);
-        __builtin_nontemporal_store(ri, r + row);
-        dot = (double)ri * (double)ri;
+    for (int rep = 0; rep < kTrueLoop; ++rep) {
+        int64_t row;
+        bool valid;
+        const double ax = sell_rowsum_of_thread<kGroupF16, __half, double, kTrueWaves>(A, x, row, valid,
+                                                                                      blockIdx.x * kTrueLoop + rep,
+                                                                                      rep == 0);
+        if (valid) {
+            const float ri = (float)(b[row] - ax);
+            __builtin_nontemporal_store(ri, r + row);
+            dot += (double)ri * (double)ri;
+        }
     }
     store_partial(block_sum<kTrueThreads>(dot, red), red_part);
 }
@@ -1102,7 +1106,7 @@ int hpccg_fcg_learned_dev(hpccg_ctx *c, int32_t nx, int32_t ny, int32_t nz, cons
     HPCCG_HIP_TRY(hipMemcpyAsync(F->hist, neg.data(), neg.size() * 8, hipMemcpyHostToDevice, c->stream));
     double *hrec = F->hist, *htrue = F->hist + F->hist_cap;
     const Matrix &A = c->A;
-    const int Gs = sell_grid(A), Gt = sell_grid(A, kTrueWaves), Gv = vec_grid_for(c->info.num_cu, n);
+    const int Gs = sell_grid(A), Gt = true_blocks(A.nslices), Gv = vec_grid_for(c->info.num_cu, n);
     const int *skip = &F->st->done;
     const float *s_in = &F->st->s_in, *s_out = &F->st->s_out;
 

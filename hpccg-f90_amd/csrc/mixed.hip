@@ -58,14 +58,18 @@ __global__ __launch_bounds__(kTrueThreads) void km_true_residual(const SellRef<_
 {
     __shared__ double red[kTrueThreads / kWave];
     if (!km_true_due<PREP>(st)) return;
-    int64_t row;
-    bool valid;
-    const double ax = sell_rowsum_of_thread<kGroupF16, __half, double, kTrueWaves>(A, x, row, valid);
     double dot = 0.0;
-    if (valid) {
-        const float ri = (float)(b[row] - ax);
-        __builtin_nontemporal_store(ri, r + row);
-        dot = (double)ri * (double)ri;
+    for (int rep = 0; rep < kTrueLoop; ++rep) {
+        int64_t row;
+        bool valid;
+        const double ax = sell_rowsum_of_thread<kGroupF16, __half, double, kTrueWaves>(A, x, row, valid,
+                                                                                      blockIdx.x * kTrueLoop + rep,
+                                                                                      rep == 0);
+        if (valid) {
+            const float ri = (float)(b[row] - ax);
+            __builtin_nontemporal_store(ri, r + row);
+            dot += (double)ri * (double)ri;
+        }
     }
     store_partial(block_sum<kTrueThreads>(dot, red), red_part);
 }
@@ -314,7 +318,7 @@ template <bool PREP>
 static void launch_km_true(hpccg_ctx *c, const double *b, const double *x)
 {
     const Matrix &A = c->A;
-    const int G = sell_grid(A, kTrueWaves);
+    const int G = true_blocks(A.nslices);
     MixedWs &m = c->mix;
     MixState *st = (MixState *)m.state;
     hipLaunchKernelGGL(km_true_residual<PREP>, dim3(G), dim3(kTrueThreads), 0, c->stream, sell16(A), x, b, m.r, st,

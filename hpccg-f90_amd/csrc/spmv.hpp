@@ -48,6 +48,10 @@ constexpr int kSellThreads = kSlice * kSellWaves;
 // 0.75 ms against 0.66 ms), and a real one costs more than all empty ones of a solve.
 constexpr int kTrueWaves = 4;
 constexpr int kTrueThreads = kSlice * kTrueWaves;
+// ... and each true-residual workgroup walks kTrueLoop consecutive groups of kTrueWaves
+// slices, so that the launches that find nothing due start 4x fewer workgroups (256^3: an
+// empty launch 15 -> 4.9 us, a real one 0.44 -> 0.48 ms; mixed CG has ~67 empty ones and 3 real)
+constexpr int kTrueLoop = 4;
 constexpr int kStencilK = 27;         // slice width of the 27-point stencil: fully unrolled path
 constexpr int kGroupF64 = 2;          // FP64 copy: 2 entries per lane per load (16-byte value loads)
 constexpr int kGroupF16 = 1;          // FP16 copy: 1 entry per lane per load (measured best)
@@ -275,15 +279,18 @@ __device__ __forceinline__ int sell_block()
 // the preconditioner's output just written, keeps hardware order (22.2 -> 22.4 ms with it).
 template <int G, typename VT, typename XT, int WAVES = kSellWaves, bool XRUN = false>
 __device__ __forceinline__ double sell_rowsum_of_thread(const SellRef<VT> &A, const XT *__restrict__ x, int64_t &row,
-                                                        bool &valid)
+                                                        bool &valid, int group = -1, bool load_table = true)
 {
-    // the offset table of a coded copy goes to LDS first (every thread of the workgroup gets here)
+    // the offset table of a coded copy goes to LDS first (every thread of the workgroup gets
+    // here; a workgroup that walks several groups loads it on the first call only)
     __shared__ int32_t tab[kMaxCodes];
-    if (A.code) {
+    if (A.code && load_table) {
         for (int i = threadIdx.x; i < A.ntab; i += blockDim.x) tab[i] = A.tab[i];
         __syncthreads();
     }
-    const int s = (XRUN ? sell_block() : (int)blockIdx.x) * WAVES + __builtin_amdgcn_readfirstlane(threadIdx.x / kSlice);
+    // slice group: `group` when given, else this workgroup's (renumbered when XRUN)
+    const int g = group >= 0 ? group : XRUN ? sell_block() : (int)blockIdx.x;
+    const int s = g * WAVES + __builtin_amdgcn_readfirstlane(threadIdx.x / kSlice);
     const int lane = threadIdx.x % kSlice;
     row = (int64_t)s * kSlice + lane;
     valid = s < A.nslices && row < A.nrow;
@@ -442,6 +449,10 @@ __device__ __forceinline__ void st4(float *p, f32x4v v) { *(f32x4v *)p = v; }
 // workgroup, in slice order (the hardware dispatches workgroups in order, so the waves in
 // flight stream a compact window of the matrix); the vector kernels use a resident grid.
 inline int sell_blocks(int64_t nslices, int waves = kSellWaves) { return (int)((nslices + waves - 1) / waves); }
+inline int true_blocks(int64_t nslices)
+{
+    return nslices > 0 ? (sell_blocks(nslices, kTrueWaves) + kTrueLoop - 1) / kTrueLoop : 1;
+}
 inline int vec_grid_for(int num_cu, int64_t n)
 {
     int64_t G = (n + kVecThreads - 1) / kVecThreads;
