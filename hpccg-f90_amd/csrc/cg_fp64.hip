@@ -861,8 +861,12 @@ static void free_matrix(hpccg_ctx *c)
 
 // Distinct offsets (col - 1) - row of a 1-based CSR, collected into an open-addressed set of
 // kOffsetSlots slots (initialised to kNoOffset), one thread per row.  *count is the number
-// inserted; the kernel stops inserting once it exceeds kMaxCodes (the copy then keeps int32
-// columns), and a full set raises *count to kOffsetSlots.
+// inserted.  Threads stop once it exceeds kMaxCodes (the copy then keeps int32 columns).
+// Concurrent threads can insert past kMaxCodes before they see that, so a probe sequence is
+// capped at kMaxProbe slots: with at most kMaxCodes entries in kOffsetSlots slots a longer
+// one means an overfull set, and it raises *count to kOffsetSlots.  Every thread thus stops
+// after a bounded number of loads, however many offsets a general matrix has.
+constexpr int kMaxProbe = 64;
 __global__ void k_column_offsets(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col, int nrow,
                                  int32_t *set, int *count)
 {
@@ -870,10 +874,11 @@ __global__ void k_column_offsets(const int32_t *__restrict__ rowptr, const int32
     if (t >= nrow) return;
     const int k1 = rowptr[t + 1] - 1;
     for (int k = rowptr[t] - 1; k < k1; ++k) {
+        if (__atomic_load_n(count, __ATOMIC_RELAXED) > kMaxCodes) return;
         const int32_t off = (col[k] - 1) - t;
         uint32_t h = ((uint32_t)off * 2654435761u) >> 20;
         int probe = 0;
-        for (; probe < kOffsetSlots; ++probe, h = (h + 1) & (kOffsetSlots - 1)) {
+        for (; probe < kMaxProbe; ++probe, h = (h + 1) & (kOffsetSlots - 1)) {
             int32_t cur = __atomic_load_n(set + h, __ATOMIC_RELAXED);
             if (cur == off) break;
             if (cur != kNoOffset) continue;
@@ -885,7 +890,7 @@ __global__ void k_column_offsets(const int32_t *__restrict__ rowptr, const int32
             }
             if (cur == off) break;
         }
-        if (probe == kOffsetSlots) {
+        if (probe == kMaxProbe) {
             atomicMax(count, kOffsetSlots);
             return;
         }

@@ -156,6 +156,25 @@ def test_spmv_generic_csr_keeps_int32_columns(ctx):
     assert np.array_equal(ctx.spmv(x), oracle.spmv(A, x))
 
 
+def test_upload_of_a_large_random_csr_falls_back_quickly(ctx):
+    """Millions of distinct offsets: collecting them stops after a bounded number of probes per
+    row (the upload stays fast), and the copy keeps int32 columns."""
+    import time
+    rng = np.random.default_rng(11)
+    n, k = 300_000, 30
+    cols = np.sort(rng.integers(0, n, (n, k)), axis=1).astype(np.int32).ravel() + 1
+    rowindex = (1 + k * np.arange(n + 1)).astype(np.int32)
+    vals = rng.standard_normal(n * k)
+    t = time.perf_counter()
+    ctx.upload(rowindex, cols, vals)
+    assert time.perf_counter() - t < 20.0
+    assert ctx.matrix_format()["ncodes"] == 0
+    A = type("P", (), {})()  # minimal oracle.Problem stand-in for oracle.spmv
+    A.n, A.rowindex, A.colindex, A.values = n, rowindex, cols, vals
+    x = rng.standard_normal(n)
+    assert np.array_equal(ctx.spmv(x), oracle.spmv(A, x))
+
+
 @pytest.mark.parametrize("n", [1, 63, 64, 65, 1000, 4099])
 def test_spmv_column_codes_banded_with_empty_rows(ctx, n):
     """At most 256 distinct offsets: one-byte column codes.  A banded CSR with 40 offsets (so
