@@ -11,8 +11,12 @@
 // slice-parallel: a wave owns `rows per wave` / 64 consecutive slices of the whole batch and
 // writes one dot partial per slice; a per-system finish kernel (one workgroup per system) sums
 // its slices' partials in a fixed order and updates that system's scalars.  A converged
-// system's slices return at once.  The whole solve -- Prepare and max_iter iterations -- is
-// captured as ONE graph and replayed with one launch.
+// system's slices return at once, and once every system has stopped every launch returns at
+// its first instruction.  Iterations run as captured graphs of 1, 2, 4, ... kBatchCap
+// iterations, enqueued back to back without a host round trip; the host reads the systems'
+// state only between launches that it sized from the batch's worst residual ratio, and a
+// repeated solve of the same batch enqueues the iteration count of the previous solve at once
+// (one launch sequence, no read until it has drained).
 //
 // Launch shape (rows per wave and waves per workgroup) is chosen per batch by
 // hpccg_batch_autotune, which times every compiled variant on the batch.
@@ -64,11 +68,15 @@ struct Batch {
     double *partials = nullptr;         // [2 NS]: per-slice dot partials
     int *nactive = nullptr;             // systems still iterating (0: every launch returns at once)
     int best_variant = -1;
-    hipGraphExec_t graph = nullptr;
-    const void *g_b = nullptr, *g_x = nullptr;
-    int g_variant = -1, g_max_iter = -1;
-    double g_tol2 = -1.0;
+    BatchState *snap = nullptr;         // pinned: [2][S] state snapshots (after Prepare, latest)
+    int hint = 0;                       // iterations the previous solve needed (0: none yet)
+    GraphCache graphs;                  // iteration graphs by (b, x, variant, size)
 };
+
+// most iterations in one captured graph; launches are split into power-of-two graphs
+constexpr int kBatchCap = 64;
+// iterations of the first launch when no previous solve gave a count
+constexpr int kBatchFirst = 16;
 
 // compiled variants: waves per workgroup x slices per wave (rows per wave = 64 x slices)
 struct Variant {
@@ -294,16 +302,22 @@ __global__ __launch_bounds__(W * 64) void kb_spmv(const BatchSell A, const float
     });
 }
 
-// one workgroup per system: alpha = rtrans / p.Ap
+// one workgroup per system: alpha = rtrans / p.Ap.  On a breakdown (p.Ap zero or not finite)
+// the system stops here: x keeps the last iterate, and it is reported unconverged.
 __global__ __launch_bounds__(256) void kb_pap_finish(const double *__restrict__ part, const int *__restrict__ slice_off,
-                                                     BatchState *st)
+                                                     BatchState *st, int *nactive)
 {
     __shared__ double red[4];
     BatchState *S = st + blockIdx.x;
     if (!S->active) return;
     const double pap = system_sum(part, slice_off, blockIdx.x, 1, 0, red);
     if (threadIdx.x == 0) {
-        if (!(pap != 0.0) || !isfinite(pap)) S->breakdown = 1;
+        if (!(pap != 0.0) || !isfinite(pap)) {
+            S->breakdown = 1;
+            S->active = 0;
+            atomicSub(nactive, 1);
+            return;
+        }
         S->alpha = S->rtrans / pap;
     }
 }
@@ -362,40 +376,45 @@ void free_batched(hpccg_ctx *c)
 {
     Batch *B = batch_of(c);
     if (!B) return;
-    if (B->graph) hipGraphExecDestroy(B->graph);
-    hipFree(B->rowindex);
-    hipFree(B->colindex);
-    hipFree(B->values);
-    hipFree(B->row_off);
-    hipFree(B->slice_ptr);
-    hipFree(B->code);
-    hipFree(B->tabs);
-    hipFree(B->sval);
-    hipFree(B->slice_off);
-    hipFree(B->r);
-    hipFree(B->p);
-    hipFree(B->Ap);
-    hipFree(B->st);
-    hipFree(B->partials);
-    hipFree(B->slice_sys);
-    hipFree(B->nactive);
+    B->graphs.clear();
+    hipHostFree(B->snap);
+    dfree(B->rowindex);
+    dfree(B->colindex);
+    dfree(B->values);
+    dfree(B->row_off);
+    dfree(B->slice_ptr);
+    dfree(B->code);
+    dfree(B->tabs);
+    dfree(B->sval);
+    dfree(B->slice_off);
+    dfree(B->r);
+    dfree(B->p);
+    dfree(B->Ap);
+    dfree(B->st);
+    dfree(B->partials);
+    dfree(B->slice_sys);
+    dfree(B->nactive);
     delete B;
     c->ext[0] = nullptr;
 }
 
+// Prepare (iters < 0: r = b - A x, ||b||, tolerances, active count) or `iters` iterations.
 template <int W, int SPW>
-static void enqueue_batch(hpccg_ctx *c, Batch *B, const double *b, double *x, int max_iter, double rel_tol2)
+static void enqueue_batch(hpccg_ctx *c, Batch *B, const double *b, double *x, int iters, double rel_tol2)
 {
     const dim3 grid((unsigned)((B->NS + W * SPW - 1) / (W * SPW))), blk(W * 64), sys((unsigned)B->S), fin(256);
     const BatchSell A = batch_sell(B);
     double *part = B->partials;
     int *na = B->nactive;
-    hipLaunchKernelGGL((kb_true<W, SPW, true>), grid, blk, 0, c->stream, A, x, b, B->r, B->st, part, na);
-    hipLaunchKernelGGL(kb_true_finish<true>, sys, fin, 0, c->stream, part, B->slice_off, B->st, rel_tol2, na);
-    for (int it = 0; it < max_iter; ++it) {
+    if (iters < 0) {
+        hipLaunchKernelGGL((kb_true<W, SPW, true>), grid, blk, 0, c->stream, A, x, b, B->r, B->st, part, na);
+        hipLaunchKernelGGL(kb_true_finish<true>, sys, fin, 0, c->stream, part, B->slice_off, B->st, rel_tol2, na);
+        return;
+    }
+    for (int it = 0; it < iters; ++it) {
         hipLaunchKernelGGL((kb_update_p<W, SPW>), grid, blk, 0, c->stream, A, B->r, B->p, B->st, na);
         hipLaunchKernelGGL((kb_spmv<W, SPW>), grid, blk, 0, c->stream, A, B->p, B->Ap, B->st, part, na);
-        hipLaunchKernelGGL(kb_pap_finish, sys, fin, 0, c->stream, part, B->slice_off, B->st);
+        hipLaunchKernelGGL(kb_pap_finish, sys, fin, 0, c->stream, part, B->slice_off, B->st, na);
         hipLaunchKernelGGL((kb_update_xr<W, SPW>), grid, blk, 0, c->stream, A, x, B->r, B->p, B->Ap, B->st, part,
                            na);
         hipLaunchKernelGGL(kb_xr_finish, sys, fin, 0, c->stream, part, B->slice_off, B->st);
@@ -404,64 +423,86 @@ static void enqueue_batch(hpccg_ctx *c, Batch *B, const double *b, double *x, in
     }
 }
 
-static int enqueue_variant(hpccg_ctx *c, Batch *B, int v, const double *b, double *x, int max_iter, double rel_tol2)
+static int enqueue_variant(hpccg_ctx *c, Batch *B, int v, const double *b, double *x, int iters, double rel_tol2)
 {
     switch (v) {
-    case 0: enqueue_batch<4, 1>(c, B, b, x, max_iter, rel_tol2); break;
-    case 1: enqueue_batch<8, 1>(c, B, b, x, max_iter, rel_tol2); break;
-    case 2: enqueue_batch<2, 1>(c, B, b, x, max_iter, rel_tol2); break;
-    case 3: enqueue_batch<16, 1>(c, B, b, x, max_iter, rel_tol2); break;
-    case 4: enqueue_batch<4, 2>(c, B, b, x, max_iter, rel_tol2); break;
-    case 5: enqueue_batch<8, 2>(c, B, b, x, max_iter, rel_tol2); break;
-    case 6: enqueue_batch<4, 4>(c, B, b, x, max_iter, rel_tol2); break;
-    case 7: enqueue_batch<8, 4>(c, B, b, x, max_iter, rel_tol2); break;
+    case 0: enqueue_batch<4, 1>(c, B, b, x, iters, rel_tol2); break;
+    case 1: enqueue_batch<8, 1>(c, B, b, x, iters, rel_tol2); break;
+    case 2: enqueue_batch<2, 1>(c, B, b, x, iters, rel_tol2); break;
+    case 3: enqueue_batch<16, 1>(c, B, b, x, iters, rel_tol2); break;
+    case 4: enqueue_batch<4, 2>(c, B, b, x, iters, rel_tol2); break;
+    case 5: enqueue_batch<8, 2>(c, B, b, x, iters, rel_tol2); break;
+    case 6: enqueue_batch<4, 4>(c, B, b, x, iters, rel_tol2); break;
+    case 7: enqueue_batch<8, 4>(c, B, b, x, iters, rel_tol2); break;
     default: set_error("batched: bad variant"); return HPCCG_ERR_ARG;
     }
     HPCCG_HIP_TRY(hipGetLastError());
     return 0;
 }
 
-// One solve: state reset (plain copies, outside the graph) + one graph launch.
+// max over the systems still iterating of r.r / (rel_tol ||b||)^2 (0 when none is)
+static double worst_ratio(const BatchState *st, int S)
+{
+    double w = 0.0;
+    for (int i = 0; i < S; ++i)
+        if (st[i].active && st[i].tol2 > 0.0) w = std::max(w, st[i].rtrans / st[i].tol2);
+    return w;
+}
+
+// One solve: state reset (plain copies), Prepare, then iteration graphs.  The first launch
+// sequence holds the previous solve's iteration count (or kBatchFirst); after it the host reads
+// the states and plans the rest from the worst system's residual rate (ChunkPlanner on
+// r.r / tol^2, whose tolerance is 1).  Launches past the last convergence cost ~50 us per
+// iteration, so the plan aims at the last one.
 static int batch_solve(hpccg_ctx *c, Batch *B, const double *b, double *x, double rel_tol, int max_iter,
                        int replace_every, int v, double *ms_out)
 {
-    std::vector<BatchState> init(B->S);
+    const int S = B->S;
+    std::vector<BatchState> init(S);
     for (auto &s : init) {
         std::memset(&s, 0, sizeof(s));
         s.replace_every = replace_every;
         s.max_iter = max_iter;
     }
-    HPCCG_HIP_TRY(hipMemcpyAsync(B->st, init.data(), B->S * sizeof(BatchState), hipMemcpyHostToDevice, c->stream));
+    HPCCG_HIP_TRY(hipMemcpyAsync(B->st, init.data(), S * sizeof(BatchState), hipMemcpyHostToDevice, c->stream));
     static const int zero = 0;
     HPCCG_HIP_TRY(hipMemcpyAsync(B->nactive, &zero, sizeof(int), hipMemcpyHostToDevice, c->stream));
     const double rel_tol2 = rel_tol * rel_tol;
-    // the graph bakes pointers, the variant, max_iter and rel_tol2: re-capture when they change
-    if (!B->graph || B->g_b != b || B->g_x != x || B->g_variant != v || B->g_max_iter != max_iter ||
-        B->g_tol2 != rel_tol2) {
-        if (B->graph) hipGraphExecDestroy(B->graph);
-        B->graph = nullptr;
-        hipGraph_t g;
-        HPCCG_HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        const int st = enqueue_variant(c, B, v, b, x, max_iter, rel_tol2);
-        const hipError_t e = hipStreamEndCapture(c->stream, &g);
-        if (st) return st;
-        HPCCG_HIP_TRY(e);
-        const hipError_t e2 = hipGraphInstantiate(&B->graph, g, nullptr, nullptr, 0);
-        hipGraphDestroy(g);
-        HPCCG_HIP_TRY(e2);
-        B->g_b = b;
-        B->g_x = x;
-        B->g_variant = v;
-        B->g_max_iter = max_iter;
-        B->g_tol2 = rel_tol2;
-    }
+    // iteration graphs bake b, x and the variant (tolerances and limits live in the states)
+    auto get = [&](int size, hipGraphExec_t *ge) {
+        return cached_graph(B->graphs, c->stream, GraphKey{b, x, 300 + v, size},
+                            [&]() { return enqueue_variant(c, B, v, b, x, size, rel_tol2); }, ge);
+    };
+    HPCCG_TRY(capture_iteration_graphs(kBatchCap, max_iter, get));
+    BatchState *prep = B->snap, *now = B->snap + S;
     HPCCG_HIP_TRY(hipEventRecord(c->ev0, c->stream));
-    HPCCG_HIP_TRY(hipGraphLaunch(B->graph, c->stream));
+    HPCCG_TRY(enqueue_variant(c, B, v, b, x, -1, rel_tol2));
+    HPCCG_HIP_TRY(hipMemcpyAsync(prep, B->st, S * sizeof(BatchState), hipMemcpyDeviceToHost, c->stream));
+    ChunkPlanner plan{kBatchCap, 1.0, 1};
+    int launched = 0, m = std::min(max_iter, B->hint > 0 ? B->hint : kBatchFirst);
+    bool first = true;
+    while (m > 0) {
+        HPCCG_TRY(launch_iterations(c->stream, m, get));
+        launched += m;
+        if (launched >= max_iter) break;
+        HPCCG_HIP_TRY(hipMemcpyAsync(now, B->st, S * sizeof(BatchState), hipMemcpyDeviceToHost, c->stream));
+        HPCCG_HIP_TRY(hipStreamSynchronize(c->stream));
+        if (first) plan.next(worst_ratio(prep, S), 0, max_iter);
+        first = false;
+        const double w = worst_ratio(now, S);
+        if (w == 0.0) break;  // every system has stopped
+        m = plan.next(w, launched, max_iter - launched);
+    }
     HPCCG_HIP_TRY(hipEventRecord(c->ev1, c->stream));
+    HPCCG_HIP_TRY(hipMemcpyAsync(now, B->st, S * sizeof(BatchState), hipMemcpyDeviceToHost, c->stream));
     HPCCG_HIP_TRY(hipEventSynchronize(c->ev1));
+    HPCCG_HIP_TRY(hipStreamSynchronize(c->stream));
     float ms = 0.f;
     HPCCG_HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     if (ms_out) *ms_out = ms;
+    int need = 0;
+    for (int i = 0; i < S; ++i) need = std::max(need, now[i].niters);
+    B->hint = need;
     return 0;
 }
 
@@ -496,13 +537,13 @@ int hpccg_batch_setup(hpccg_ctx *c, int32_t count, const int32_t *dims)
         set_error("batch too large for int32 CSR");
         return HPCCG_ERR_ARG;
     }
-    if (hipMalloc(&B->rowindex, (B->E + 1) * sizeof(int32_t)) != hipSuccess ||
-        hipMalloc(&B->colindex, B->nnz * sizeof(int32_t)) != hipSuccess ||
-        hipMalloc(&B->values, B->nnz * sizeof(__half)) != hipSuccess ||
-        hipMalloc(&B->row_off, (count + 1) * sizeof(int64_t)) != hipSuccess ||
-        hipMalloc(&B->r, B->E * sizeof(float)) != hipSuccess || hipMalloc(&B->p, B->E * sizeof(float)) != hipSuccess ||
-        hipMalloc(&B->Ap, B->E * sizeof(float)) != hipSuccess ||
-        hipMalloc(&B->st, count * sizeof(BatchState)) != hipSuccess) {
+    if (dmalloc(&B->rowindex, (B->E + 1) * sizeof(int32_t)) != hipSuccess ||
+        dmalloc(&B->colindex, B->nnz * sizeof(int32_t)) != hipSuccess ||
+        dmalloc(&B->values, B->nnz * sizeof(__half)) != hipSuccess ||
+        dmalloc(&B->row_off, (count + 1) * sizeof(int64_t)) != hipSuccess ||
+        dmalloc(&B->r, B->E * sizeof(float)) != hipSuccess || dmalloc(&B->p, B->E * sizeof(float)) != hipSuccess ||
+        dmalloc(&B->Ap, B->E * sizeof(float)) != hipSuccess ||
+        dmalloc(&B->st, count * sizeof(BatchState)) != hipSuccess) {
         free_batched(c);
         set_error("batched: device allocation failed");
         return HPCCG_ERR_NOMEM;
@@ -528,11 +569,11 @@ int hpccg_batch_setup(hpccg_ctx *c, int32_t count, const int32_t *dims)
         sell_slice_ptr(rowh.data() + B->row_off_h[s], B->row_off_h[s + 1] - B->row_off_h[s], sp[soff[s]],
                        sp.data() + soff[s]);
     B->slots = sp[soff[count]];
-    if (hipMalloc(&B->slice_ptr, sp.size() * sizeof(int64_t)) != hipSuccess ||
-        hipMalloc(&B->code, std::max<int64_t>(code_bytes(B->slots, soff[count]), 4)) != hipSuccess ||
-        hipMalloc(&B->tabs, (size_t)count * kBatchTab * sizeof(int32_t)) != hipSuccess ||
-        hipMalloc(&B->sval, std::max<int64_t>(B->slots, 1) * sizeof(__half)) != hipSuccess ||
-        hipMalloc(&B->slice_off, (count + 1) * sizeof(int)) != hipSuccess) {
+    if (dmalloc(&B->slice_ptr, sp.size() * sizeof(int64_t)) != hipSuccess ||
+        dmalloc(&B->code, std::max<int64_t>(code_bytes(B->slots, soff[count]), 4)) != hipSuccess ||
+        dmalloc(&B->tabs, (size_t)count * kBatchTab * sizeof(int32_t)) != hipSuccess ||
+        dmalloc(&B->sval, std::max<int64_t>(B->slots, 1) * sizeof(__half)) != hipSuccess ||
+        dmalloc(&B->slice_off, (count + 1) * sizeof(int)) != hipSuccess) {
         free_batched(c);
         set_error("batched: device allocation of the sliced copy failed");
         return HPCCG_ERR_NOMEM;
@@ -541,9 +582,10 @@ int hpccg_batch_setup(hpccg_ctx *c, int32_t count, const int32_t *dims)
     std::vector<int> ssys(std::max(B->NS, 1), 0);
     for (int s = 0; s < count; ++s)
         for (int i = soff[s]; i < soff[s + 1]; ++i) ssys[i] = s;
-    if (hipMalloc(&B->slice_sys, ssys.size() * sizeof(int)) != hipSuccess ||
-        hipMalloc(&B->partials, 2 * ssys.size() * sizeof(double)) != hipSuccess ||
-        hipMalloc(&B->nactive, sizeof(int)) != hipSuccess) {
+    if (dmalloc(&B->slice_sys, ssys.size() * sizeof(int)) != hipSuccess ||
+        dmalloc(&B->partials, 2 * ssys.size() * sizeof(double)) != hipSuccess ||
+        dmalloc(&B->nactive, sizeof(int)) != hipSuccess ||
+        hipHostMalloc(&B->snap, 2 * (size_t)count * sizeof(BatchState), hipHostMallocDefault) != hipSuccess) {
         free_batched(c);
         set_error("batched: device allocation failed");
         return HPCCG_ERR_NOMEM;
@@ -566,7 +608,7 @@ int hpccg_batch_setup(hpccg_ctx *c, int32_t count, const int32_t *dims)
         for (int i = 0; i < kBatchTab; ++i) tabs[(size_t)s * kBatchTab + i] = t[std::min(i, ntab[s] - 1)];
     }
     int *flag = nullptr;
-    HPCCG_HIP_TRY(hipMalloc(&flag, sizeof(int)));
+    HPCCG_HIP_TRY(dmalloc(&flag, sizeof(int)));
     int bad = 0;
     hipError_t e = hipMemcpyAsync(B->tabs, tabs.data(), tabs.size() * sizeof(int32_t), hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(flag, 0, sizeof(int), c->stream);
@@ -589,7 +631,7 @@ int hpccg_batch_setup(hpccg_ctx *c, int32_t count, const int32_t *dims)
     }
     if (e == hipSuccess) e = hipMemcpyAsync(&bad, flag, sizeof(int), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    hipFree(flag);
+    dfree(flag);
     if (e != hipSuccess || bad) {
         free_batched(c);
         set_error(e != hipSuccess ? "batched: sliced copy: %s" : "batched: a column offset is missing from its table%s",
@@ -662,7 +704,7 @@ int hpccg_batch_autotune(hpccg_ctx *c, const double *b, double *x, int32_t probe
         return HPCCG_ERR_STATE;
     }
     double *x0 = nullptr;
-    HPCCG_HIP_TRY(hipMalloc(&x0, B->E * sizeof(double)));
+    HPCCG_HIP_TRY(dmalloc(&x0, B->E * sizeof(double)));
     HPCCG_HIP_TRY(hipMemcpyAsync(x0, x, B->E * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     // Score = time of probe_iters working iterations + time of probe_iters iterations after
     // every system has converged (rel_tol 1e100 stops them all at Prepare).  A solve captures
@@ -687,7 +729,7 @@ int hpccg_batch_autotune(hpccg_ctx *c, const double *b, double *x, int32_t probe
         score[v] += idle;
     }
     if (st) {
-        hipFree(x0);
+        dfree(x0);
         return st;
     }
     int bi = 0;
@@ -697,9 +739,9 @@ int hpccg_batch_autotune(hpccg_ctx *c, const double *b, double *x, int32_t probe
     }
     HPCCG_HIP_TRY(hipMemcpyAsync(x, x0, B->E * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     HPCCG_HIP_TRY(hipStreamSynchronize(c->stream));
-    hipFree(x0);
-    if (B->graph) hipGraphExecDestroy(B->graph);
-    B->graph = nullptr;
+    dfree(x0);
+    B->graphs.clear();
+    B->hint = 0;  // the probes' counts say nothing about a real solve
     B->best_variant = bi;
     if (best) *best = bi;
     return 0;

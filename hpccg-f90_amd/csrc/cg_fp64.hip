@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
@@ -24,6 +25,22 @@
 namespace hpccg {
 
 static thread_local char g_err[512] = "";
+static std::atomic<uint64_t> g_buffer_gen{1};
+
+uint64_t buffer_generation() { return g_buffer_gen.load(std::memory_order_acquire); }
+
+hipError_t dmalloc_bytes(void **p, size_t bytes)
+{
+    g_buffer_gen.fetch_add(1, std::memory_order_acq_rel);
+    return hipMalloc(p, bytes);
+}
+
+hipError_t dfree(void *p)
+{
+    if (!p) return hipSuccess;
+    g_buffer_gen.fetch_add(1, std::memory_order_acq_rel);
+    return hipFree(p);
+}
 
 void set_error(const char *fmt, ...)
 {
@@ -407,15 +424,13 @@ static int vec_grid(const hpccg_ctx *c, int64_t n) { return vec_grid_for(c->info
 static int ensure_workspace(hpccg_ctx *c, int64_t n)
 {
     if (c->ws_n >= n && c->r) return 0;
-    hipFree(c->r);
-    hipFree(c->p);
-    hipFree(c->Ap);
+    dfree(c->r);
+    dfree(c->p);
+    dfree(c->Ap);
     c->r = c->p = c->Ap = nullptr;
     c->ws_n = 0;
-    for (auto &g : c->graphs) hipGraphExecDestroy(g.second);
-    c->graphs.clear();
-    if (hipMalloc(&c->r, n * sizeof(double)) != hipSuccess || hipMalloc(&c->p, n * sizeof(double)) != hipSuccess ||
-        hipMalloc(&c->Ap, n * sizeof(double)) != hipSuccess) {
+    if (dmalloc(&c->r, n * sizeof(double)) != hipSuccess || dmalloc(&c->p, n * sizeof(double)) != hipSuccess ||
+        dmalloc(&c->Ap, n * sizeof(double)) != hipSuccess) {
         set_error("ensure_workspace: hipMalloc of %lld doubles failed", (long long)(3 * n));
         return HPCCG_ERR_NOMEM;
     }
@@ -426,13 +441,13 @@ static int ensure_workspace(hpccg_ctx *c, int64_t n)
 static int ensure_stage(hpccg_ctx *c, int64_t n)
 {
     if (c->stage_n >= n && c->s0) return 0;
-    hipFree(c->s0);
-    hipFree(c->s1);
-    hipFree(c->s2);
+    dfree(c->s0);
+    dfree(c->s1);
+    dfree(c->s2);
     c->s0 = c->s1 = c->s2 = nullptr;
     c->stage_n = 0;
-    if (hipMalloc(&c->s0, n * sizeof(double)) != hipSuccess || hipMalloc(&c->s1, n * sizeof(double)) != hipSuccess ||
-        hipMalloc(&c->s2, n * sizeof(double)) != hipSuccess) {
+    if (dmalloc(&c->s0, n * sizeof(double)) != hipSuccess || dmalloc(&c->s1, n * sizeof(double)) != hipSuccess ||
+        dmalloc(&c->s2, n * sizeof(double)) != hipSuccess) {
         set_error("ensure_stage: hipMalloc failed");
         return HPCCG_ERR_NOMEM;
     }
@@ -443,12 +458,10 @@ static int ensure_stage(hpccg_ctx *c, int64_t n)
 static int ensure_hist(hpccg_ctx *c, int need)
 {
     if (c->hist_cap >= need) return 0;
-    hipFree(c->rr_hist);
+    dfree(c->rr_hist);
     c->rr_hist = nullptr;
     c->hist_cap = 0;
-    for (auto &g : c->graphs) hipGraphExecDestroy(g.second);
-    c->graphs.clear();
-    if (hipMalloc(&c->rr_hist, (size_t)need * sizeof(double)) != hipSuccess) {
+    if (dmalloc(&c->rr_hist, (size_t)need * sizeof(double)) != hipSuccess) {
         set_error("ensure_hist: hipMalloc failed");
         return HPCCG_ERR_NOMEM;
     }
@@ -530,31 +543,8 @@ static int enqueue_iterations(hpccg_ctx *c, double *x, int count)
 
 static int get_chunk_graph(hpccg_ctx *c, const double *b, double *x, int chunk, hipGraphExec_t *out)
 {
-    GraphKey key{b, x, c->dot_mode, chunk};
-    auto it = c->graphs.find(key);
-    if (it != c->graphs.end()) {
-        *out = it->second;
-        return 0;
-    }
-    hipGraph_t g;
-    HPCCG_HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    int st = enqueue_iterations(c, x, chunk);
-    hipError_t e = hipStreamEndCapture(c->stream, &g);
-    if (st) return st;
-    if (e != hipSuccess) {
-        set_error("hipStreamEndCapture: %s", hipGetErrorString(e));
-        return HPCCG_ERR_HIP;
-    }
-    hipGraphExec_t ge;
-    e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-    hipGraphDestroy(g);
-    if (e != hipSuccess) {
-        set_error("hipGraphInstantiate: %s", hipGetErrorString(e));
-        return HPCCG_ERR_HIP;
-    }
-    c->graphs[key] = ge;
-    *out = ge;
-    return 0;
+    return cached_graph(c->graphs, c->stream, GraphKey{b, x, c->dot_mode, chunk},
+                        [&]() { return enqueue_iterations(c, x, chunk); }, out);
 }
 
 // The whole solve on device-resident b and x.
@@ -720,12 +710,13 @@ int hpccg_create(hpccg_ctx **out, int device)
     c->info.num_cu = prop.multiProcessorCount;
     c->info.gfx = 950;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipMalloc(&c->state, sizeof(CgState)) != hipSuccess ||
+        dmalloc(&c->state, sizeof(CgState)) != hipSuccess ||
         hipHostMalloc(&c->state_h, sizeof(CgState), hipHostMallocDefault) != hipSuccess ||
-        hipMalloc(&c->partials, kMaxGrid * sizeof(double)) != hipSuccess ||
-        hipMalloc(&c->scalar, 8 * sizeof(double)) != hipSuccess ||
+        dmalloc(&c->partials, kMaxGrid * sizeof(double)) != hipSuccess ||
+        dmalloc(&c->scalar, 8 * sizeof(double)) != hipSuccess ||
         hipHostMalloc(&c->scalar_h, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
-        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipEventCreate(&c->ev2) != hipSuccess || hipEventCreate(&c->ev3) != hipSuccess) {
         set_error("hpccg_create: HIP resource allocation failed");
         hpccg_destroy(c);
         return HPCCG_ERR_HIP;
@@ -752,20 +743,22 @@ int hpccg_destroy(hpccg_ctx *c)
     if (c->stream) hipStreamSynchronize(c->stream);
     free_matrix(c);  // also drops every captured graph
     hpccg_free_ext(c);
-    hipFree(c->r);
-    hipFree(c->p);
-    hipFree(c->Ap);
-    hipFree(c->state);
+    dfree(c->r);
+    dfree(c->p);
+    dfree(c->Ap);
+    dfree(c->state);
     hipHostFree(c->state_h);
-    hipFree(c->rr_hist);
-    hipFree(c->partials);
-    hipFree(c->scalar);
+    dfree(c->rr_hist);
+    dfree(c->partials);
+    dfree(c->scalar);
     hipHostFree(c->scalar_h);
-    hipFree(c->s0);
-    hipFree(c->s1);
-    hipFree(c->s2);
+    dfree(c->s0);
+    dfree(c->s1);
+    dfree(c->s2);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->ev2) hipEventDestroy(c->ev2);
+    if (c->ev3) hipEventDestroy(c->ev3);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return 0;
@@ -810,6 +803,7 @@ int hpccg_dev_free(hpccg_ctx *c, void *ptr)
 {
     HPCCG_CHECK_ARG(c, "null context");
     HPCCG_HIP_TRY(hipStreamSynchronize(c->stream));
+    // caller-owned vectors are part of the graph keys, not of the buffer generation
     HPCCG_HIP_TRY(hipFree(ptr));
     return 0;
 }
@@ -838,12 +832,11 @@ int hpccg_memcpy_d2d(hpccg_ctx *c, void *dst, const void *src, int64_t bytes)
     return 0;
 }
 
-// Every captured graph of every path holds the matrix addresses: drop them with the matrix.
+// Every captured graph of every path holds the matrix addresses.  Freeing the matrix bumps the
+// buffer generation, which already retires them; dropping them here releases their memory.
 static void drop_graphs(hpccg_ctx *c)
 {
-    for (auto &g : c->graphs) hipGraphExecDestroy(g.second);
     c->graphs.clear();
-    for (auto &g : c->mix.graphs) hipGraphExecDestroy(g.second);
     c->mix.graphs.clear();
     invalidate_learned_graphs(c);
 }
@@ -855,7 +848,7 @@ static void free_matrix(hpccg_ctx *c)
     for (void *p : {(void *)A.rowindex, (void *)A.colindex, (void *)A.values, (void *)A.slice_ptr, (void *)A.code,
                     (void *)A.ctab, (void *)A.scol64, (void *)A.sval64, (void *)A.scol16, (void *)A.sval16,
                     (void *)A.red_part})
-        hipFree(p);
+        dfree(p);
     A = Matrix{};
 }
 
@@ -905,7 +898,7 @@ static hipError_t column_offset_table(hpccg_ctx *c, std::vector<int32_t> &tab)
     tab.clear();
     if (A.nrow == 0 || A.nnz == 0) return hipSuccess;
     int32_t *set = nullptr;  // kOffsetSlots slots, then the count
-    hipError_t e = hipMalloc(&set, (kOffsetSlots + 1) * sizeof(int32_t));
+    hipError_t e = dmalloc(&set, (kOffsetSlots + 1) * sizeof(int32_t));
     if (e != hipSuccess) return e;
     std::vector<int32_t> h(kOffsetSlots + 1);
     e = hipMemsetD32Async(set, kNoOffset, kOffsetSlots, c->stream);
@@ -917,7 +910,7 @@ static hipError_t column_offset_table(hpccg_ctx *c, std::vector<int32_t> &tab)
     }
     if (e == hipSuccess) e = hipMemcpyAsync(h.data(), set, h.size() * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    hipFree(set);
+    dfree(set);
     if (e != hipSuccess || h[kOffsetSlots] > kMaxCodes) return e;
     for (int i = 0; i < kOffsetSlots; ++i)
         if (h[i] != kNoOffset) tab.push_back(h[i]);
@@ -951,13 +944,13 @@ static int finish_matrix_setup(hpccg_ctx *c, const int32_t *rowindex_host)
     const int64_t cbytes = code_bytes(A.slots, A.nslices);
     const int G = sell_grid(A);
     int *flag = nullptr;
-    if (hipMalloc(&A.slice_ptr, sp.size() * sizeof(int64_t)) != hipSuccess ||
-        (coded && hipMalloc(&A.code, std::max<int64_t>(cbytes, 4)) != hipSuccess) ||
-        (coded && hipMalloc(&A.ctab, kMaxCodes * sizeof(int32_t)) != hipSuccess) ||
-        (!coded && hipMalloc(&A.scol64, std::max<int64_t>(A.slots, 1) * sizeof(int32_t)) != hipSuccess) ||
-        hipMalloc(&A.sval64, std::max<int64_t>(A.slots, 1) * sizeof(double)) != hipSuccess ||
-        hipMalloc(&A.red_part, (size_t)G * sizeof(double)) != hipSuccess || hipMalloc(&flag, sizeof(int)) != hipSuccess) {
-        hipFree(flag);
+    if (dmalloc(&A.slice_ptr, sp.size() * sizeof(int64_t)) != hipSuccess ||
+        (coded && dmalloc(&A.code, std::max<int64_t>(cbytes, 4)) != hipSuccess) ||
+        (coded && dmalloc(&A.ctab, kMaxCodes * sizeof(int32_t)) != hipSuccess) ||
+        (!coded && dmalloc(&A.scol64, std::max<int64_t>(A.slots, 1) * sizeof(int32_t)) != hipSuccess) ||
+        dmalloc(&A.sval64, std::max<int64_t>(A.slots, 1) * sizeof(double)) != hipSuccess ||
+        dmalloc(&A.red_part, (size_t)G * sizeof(double)) != hipSuccess || dmalloc(&flag, sizeof(int)) != hipSuccess) {
+        dfree(flag);
         free_matrix(c);
         set_error("matrix setup: device allocation of the sliced copy failed (%lld slots)", (long long)sp.back());
         return HPCCG_ERR_NOMEM;
@@ -981,7 +974,7 @@ static int finish_matrix_setup(hpccg_ctx *c, const int32_t *rowindex_host)
     int bad = 0;
     if (e == hipSuccess) e = hipMemcpyAsync(&bad, flag, sizeof(int), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    hipFree(flag);
+    dfree(flag);
     if (e != hipSuccess || bad) {
         free_matrix(c);
         if (e != hipSuccess) {
@@ -1013,9 +1006,9 @@ int hpccg_matrix_upload(hpccg_ctx *c, int32_t nrow, int32_t ncol, const int32_t 
     c->A.nrow = nrow;
     c->A.ncol = ncol;
     c->A.nnz = nnz;
-    if (hipMalloc(&c->A.rowindex, (nrow + 1) * sizeof(int32_t)) != hipSuccess ||
-        hipMalloc(&c->A.colindex, std::max<int64_t>(nnz, 1) * sizeof(int32_t)) != hipSuccess ||
-        hipMalloc(&c->A.values, std::max<int64_t>(nnz, 1) * sizeof(double)) != hipSuccess) {
+    if (dmalloc(&c->A.rowindex, (nrow + 1) * sizeof(int32_t)) != hipSuccess ||
+        dmalloc(&c->A.colindex, std::max<int64_t>(nnz, 1) * sizeof(int32_t)) != hipSuccess ||
+        dmalloc(&c->A.values, std::max<int64_t>(nnz, 1) * sizeof(double)) != hipSuccess) {
         free_matrix(c);
         set_error("hpccg_matrix_upload: device allocation failed (nnz=%lld)", (long long)nnz);
         return HPCCG_ERR_NOMEM;
@@ -1046,14 +1039,14 @@ int hpccg_generate_matrix(hpccg_ctx *c, int32_t nx, int32_t ny, int32_t nz, int3
     c->A.grid[1] = ny;
     c->A.grid[2] = nz;
     double *dx = nullptr, *db = nullptr, *dxe = nullptr;
-    if (hipMalloc(&c->A.rowindex, (N + 1) * sizeof(int32_t)) != hipSuccess ||
-        hipMalloc(&c->A.colindex, nnz * sizeof(int32_t)) != hipSuccess ||
-        hipMalloc(&c->A.values, nnz * sizeof(double)) != hipSuccess ||
-        hipMalloc(&dx, N * sizeof(double)) != hipSuccess || hipMalloc(&db, N * sizeof(double)) != hipSuccess ||
-        hipMalloc(&dxe, N * sizeof(double)) != hipSuccess) {
-        hipFree(dx);
-        hipFree(db);
-        hipFree(dxe);
+    if (dmalloc(&c->A.rowindex, (N + 1) * sizeof(int32_t)) != hipSuccess ||
+        dmalloc(&c->A.colindex, nnz * sizeof(int32_t)) != hipSuccess ||
+        dmalloc(&c->A.values, nnz * sizeof(double)) != hipSuccess ||
+        dmalloc(&dx, N * sizeof(double)) != hipSuccess || dmalloc(&db, N * sizeof(double)) != hipSuccess ||
+        dmalloc(&dxe, N * sizeof(double)) != hipSuccess) {
+        dfree(dx);
+        dfree(db);
+        dfree(dxe);
         free_matrix(c);
         set_error("hpccg_generate_matrix: device allocation failed");
         return HPCCG_ERR_NOMEM;
@@ -1072,9 +1065,9 @@ int hpccg_generate_matrix(hpccg_ctx *c, int32_t nx, int32_t ny, int32_t nz, int3
     if (e == hipSuccess && b) e = hipMemcpyAsync(b, db, N * sizeof(double), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess && xexact) e = hipMemcpyAsync(xexact, dxe, N * sizeof(double), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    hipFree(dx);
-    hipFree(db);
-    hipFree(dxe);
+    dfree(dx);
+    dfree(db);
+    dfree(dxe);
     if (e != hipSuccess) {
         set_error("hpccg_generate_matrix: %s", hipGetErrorString(e));
         return HPCCG_ERR_HIP;

@@ -121,13 +121,13 @@ static int ensure_net(hpccg_ctx *c)
     if (net_of(c)) return 0;
     CnnNet *N = new CnnNet();
     c->ext[1] = N;
-    if (hipMalloc(&N->W, NWALL * 4) != hipSuccess || hipMalloc(&N->m, NWALL * 4) != hipSuccess ||
-        hipMalloc(&N->v, NWALL * 4) != hipSuccess || hipMalloc(&N->grad, NWALL * 4) != hipSuccess ||
-        hipMalloc(&N->pk_f1, 64 * 16) != hipSuccess || hipMalloc(&N->pk_b3, 64 * 16) != hipSuccess ||
-        hipMalloc(&N->pk_f2, NSTEP16 * 64 * 16) != hipSuccess ||
-        hipMalloc(&N->pk_f3, NSTEP16 * 64 * 16) != hipSuccess || hipMalloc(&N->pk_f3f, NSTEP3F * 64 * 16) != hipSuccess ||
-        hipMalloc(&N->pk_b2, NSTEP16 * 64 * 16) != hipSuccess || hipMalloc(&N->step, sizeof(int)) != hipSuccess ||
-        hipMalloc(&N->scale_in, 4096 * 4) != hipSuccess || hipMalloc(&N->scale_out, 4096 * 4) != hipSuccess) {
+    if (dmalloc(&N->W, NWALL * 4) != hipSuccess || dmalloc(&N->m, NWALL * 4) != hipSuccess ||
+        dmalloc(&N->v, NWALL * 4) != hipSuccess || dmalloc(&N->grad, NWALL * 4) != hipSuccess ||
+        dmalloc(&N->pk_f1, 64 * 16) != hipSuccess || dmalloc(&N->pk_b3, 64 * 16) != hipSuccess ||
+        dmalloc(&N->pk_f2, NSTEP16 * 64 * 16) != hipSuccess ||
+        dmalloc(&N->pk_f3, NSTEP16 * 64 * 16) != hipSuccess || dmalloc(&N->pk_f3f, NSTEP3F * 64 * 16) != hipSuccess ||
+        dmalloc(&N->pk_b2, NSTEP16 * 64 * 16) != hipSuccess || dmalloc(&N->step, sizeof(int)) != hipSuccess ||
+        dmalloc(&N->scale_in, 4096 * 4) != hipSuccess || dmalloc(&N->scale_out, 4096 * 4) != hipSuccess) {
         set_error("cnn: allocation failed");
         return HPCCG_ERR_NOMEM;
     }
@@ -154,33 +154,33 @@ static int ensure_act(hpccg_ctx *c, int64_t vox, int wg)
 {
     CnnNet *N = net_of(c);
     if (vox > N->cap_vox) {
-        hipFree(N->h1);
-        hipFree(N->h2);
-        hipFree(N->dh1);
-        hipFree(N->dh2);
-        hipFree(N->dz);
-        hipFree(N->in);
-        hipFree(N->z);
-        hipFree(N->tgt);
-        hipFree(N->wbuf);
+        dfree(N->h1);
+        dfree(N->h2);
+        dfree(N->dh1);
+        dfree(N->dh2);
+        dfree(N->dz);
+        dfree(N->in);
+        dfree(N->z);
+        dfree(N->tgt);
+        dfree(N->wbuf);
         N->cap_vox = 0;
-        if (hipMalloc(&N->h1, vox * 32) != hipSuccess || hipMalloc(&N->h2, vox * 32) != hipSuccess ||
-            hipMalloc(&N->dh1, vox * 32) != hipSuccess || hipMalloc(&N->dh2, vox * 32) != hipSuccess ||
-            hipMalloc(&N->dz, vox * 2) != hipSuccess || hipMalloc(&N->in, vox * 4) != hipSuccess ||
-            hipMalloc(&N->z, vox * 4) != hipSuccess || hipMalloc(&N->tgt, vox * 4) != hipSuccess ||
-            hipMalloc(&N->wbuf, vox * 4) != hipSuccess) {
+        if (dmalloc(&N->h1, vox * 32) != hipSuccess || dmalloc(&N->h2, vox * 32) != hipSuccess ||
+            dmalloc(&N->dh1, vox * 32) != hipSuccess || dmalloc(&N->dh2, vox * 32) != hipSuccess ||
+            dmalloc(&N->dz, vox * 2) != hipSuccess || dmalloc(&N->in, vox * 4) != hipSuccess ||
+            dmalloc(&N->z, vox * 4) != hipSuccess || dmalloc(&N->tgt, vox * 4) != hipSuccess ||
+            dmalloc(&N->wbuf, vox * 4) != hipSuccess) {
             set_error("cnn: activation allocation failed (%lld voxels)", (long long)vox);
             return HPCCG_ERR_NOMEM;
         }
         N->cap_vox = vox;
         if (N->train_graph) hipGraphExecDestroy(N->train_graph);
         N->train_graph = nullptr;
-        invalidate_learned_graphs(c);  // captured FCG graphs hold the old activation buffers
+        invalidate_learned_graphs(c);  // graphs captured with the old activations (already stale)
     }
     if (wg > N->partials_wg) {
-        hipFree(N->partials);
+        dfree(N->partials);
         N->partials = nullptr;
-        HPCCG_HIP_TRY(hipMalloc(&N->partials, (size_t)wg * 16 * 16 * 27 * 4));
+        HPCCG_HIP_TRY(dmalloc(&N->partials, (size_t)wg * 16 * 16 * 27 * 4));
         N->partials_wg = wg;
         if (N->train_graph) hipGraphExecDestroy(N->train_graph);
         N->train_graph = nullptr;
@@ -668,17 +668,14 @@ struct FcgWs {
     int hist_cap = 0;
     uint16_t *h1 = nullptr, *h2 = nullptr;
     int gdims[3] = {0, 0, 0};  // grid the cached graphs were captured for
-    std::map<GraphKey, hipGraphExec_t> graphs;
+    GraphCache graphs;
 };
 
 static FcgWs *fcg_of(hpccg_ctx *c) { return (FcgWs *)c->ext[2]; }
 
 void invalidate_learned_graphs(hpccg_ctx *c)
 {
-    if (FcgWs *F = fcg_of(c)) {
-        for (auto &g : F->graphs) hipGraphExecDestroy(g.second);
-        F->graphs.clear();
-    }
+    if (FcgWs *F = fcg_of(c)) F->graphs.clear();
 }
 
 void free_learned(hpccg_ctx *c)
@@ -691,21 +688,21 @@ void free_learned(hpccg_ctx *c)
                         (void *)N->tgt, (void *)N->wbuf, (void *)N->scale_in, (void *)N->scale_out,
                         (void *)N->partials, (void *)N->grad, (void *)N->loss, (void *)N->lpart,
                         (void *)N->batch_meta, (void *)N->data.r, (void *)N->data.e, (void *)N->data.margin})
-            hipFree(p);
+            dfree(p);
         delete N;
         c->ext[1] = nullptr;
     }
     if (FcgWs *F = fcg_of(c)) {
-        for (auto &g : F->graphs) hipGraphExecDestroy(g.second);
-        hipFree(F->r);
-        hipFree(F->p);
-        hipFree(F->Ap);
-        hipFree(F->z);
-        hipFree(F->st);
+        F->graphs.clear();
+        dfree(F->r);
+        dfree(F->p);
+        dfree(F->Ap);
+        dfree(F->z);
+        dfree(F->st);
         hipHostFree(F->st_h);
-        hipFree(F->hist);
-        hipFree(F->h1);
-        hipFree(F->h2);
+        dfree(F->hist);
+        dfree(F->h1);
+        dfree(F->h2);
         delete F;
         c->ext[2] = nullptr;
     }
@@ -843,13 +840,13 @@ int hpccg_cnn_make_dataset(hpccg_ctx *c, int32_t n, int32_t npairs, int32_t crop
     HPCCG_TRY(ensure_net(c));
     CnnNet *N = net_of(c);
     TrainData &D = N->data;
-    hipFree(D.r);
-    hipFree(D.e);
-    hipFree(D.margin);
+    dfree(D.r);
+    dfree(D.e);
+    dfree(D.margin);
     D = TrainData{};
     const int64_t V = (int64_t)crop * crop * crop;
-    if (hipMalloc(&D.r, npairs * V * 4) != hipSuccess || hipMalloc(&D.e, npairs * V * 4) != hipSuccess ||
-        hipMalloc(&D.margin, npairs * 6 * sizeof(int)) != hipSuccess) {
+    if (dmalloc(&D.r, npairs * V * 4) != hipSuccess || dmalloc(&D.e, npairs * V * 4) != hipSuccess ||
+        dmalloc(&D.margin, npairs * 6 * sizeof(int)) != hipSuccess) {
         set_error("make_dataset: allocation failed");
         return HPCCG_ERR_NOMEM;
     }
@@ -857,10 +854,10 @@ int hpccg_cnn_make_dataset(hpccg_ctx *c, int32_t n, int32_t npairs, int32_t crop
     HPCCG_TRY(hpccg_generate_matrix(c, n, n, n, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr));
     const int64_t NN = (int64_t)n * n * n;
     double *xs = nullptr, *b = nullptr, *x = nullptr;
-    if (hipMalloc(&xs, NN * 8) != hipSuccess || hipMalloc(&b, NN * 8) != hipSuccess ||
-        hipMalloc(&x, NN * 8) != hipSuccess) {
-        hipFree(xs);
-        hipFree(b);
+    if (dmalloc(&xs, NN * 8) != hipSuccess || dmalloc(&b, NN * 8) != hipSuccess ||
+        dmalloc(&x, NN * 8) != hipSuccess) {
+        dfree(xs);
+        dfree(b);
         set_error("make_dataset: allocation failed");
         return HPCCG_ERR_NOMEM;
     }
@@ -913,9 +910,9 @@ int hpccg_cnn_make_dataset(hpccg_ctx *c, int32_t n, int32_t npairs, int32_t crop
     if (st == 0 && hipMemcpy(D.margin, margins.data(), margins.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
         st = HPCCG_ERR_HIP;
     if (st == 0 && hipStreamSynchronize(c->stream) != hipSuccess) st = HPCCG_ERR_HIP;
-    hipFree(xs);
-    hipFree(b);
-    hipFree(x);
+    dfree(xs);
+    dfree(b);
+    dfree(x);
     if (st) {
         if (st == HPCCG_ERR_HIP) set_error("make_dataset: HIP failure");
         return st;
@@ -958,15 +955,15 @@ int hpccg_cnn_train(hpccg_ctx *c, int32_t steps, int32_t batch, float lr, uint64
     const int slabs = std::max(1, C / 4);
     HPCCG_TRY(ensure_act(c, V * batch, wgrad_wg(c, g, batch)));
     if (N->loss_cap < steps) {
-        hipFree(N->loss);
+        dfree(N->loss);
         N->loss = nullptr;
-        HPCCG_HIP_TRY(hipMalloc(&N->loss, (size_t)steps * 4));
+        HPCCG_HIP_TRY(dmalloc(&N->loss, (size_t)steps * 4));
         N->loss_cap = steps;
     }
-    hipFree(N->lpart);
-    hipFree(N->batch_meta);
-    HPCCG_HIP_TRY(hipMalloc(&N->lpart, (size_t)2 * batch * slabs * 8));
-    HPCCG_HIP_TRY(hipMalloc(&N->batch_meta, (size_t)batch * 8 * sizeof(int)));
+    dfree(N->lpart);
+    dfree(N->batch_meta);
+    HPCCG_HIP_TRY(dmalloc(&N->lpart, (size_t)2 * batch * slabs * 8));
+    HPCCG_HIP_TRY(dmalloc(&N->batch_meta, (size_t)batch * 8 * sizeof(int)));
     int z0 = 0;
     HPCCG_HIP_TRY(hipMemcpy(N->step, &z0, sizeof(int), hipMemcpyHostToDevice));
     std::vector<float> zero(NWALL, 0.f);
@@ -1069,30 +1066,26 @@ int hpccg_fcg_learned_dev(hpccg_ctx *c, int32_t nx, int32_t ny, int32_t nz, cons
         F->gdims[2] = nz;
     }
     if (F->n < n) {
-        for (auto &gg : F->graphs) hipGraphExecDestroy(gg.second);
-        F->graphs.clear();
-        hipFree(F->r);
-        hipFree(F->p);
-        hipFree(F->Ap);
-        hipFree(F->z);
+        dfree(F->r);
+        dfree(F->p);
+        dfree(F->Ap);
+        dfree(F->z);
         F->n = 0;
-        if (hipMalloc(&F->r, n * 4) != hipSuccess || hipMalloc(&F->p, n * 4) != hipSuccess ||
-            hipMalloc(&F->Ap, n * 4) != hipSuccess || hipMalloc(&F->z, n * 4) != hipSuccess) {
+        if (dmalloc(&F->r, n * 4) != hipSuccess || dmalloc(&F->p, n * 4) != hipSuccess ||
+            dmalloc(&F->Ap, n * 4) != hipSuccess || dmalloc(&F->z, n * 4) != hipSuccess) {
             set_error("fcg: allocation failed");
             return HPCCG_ERR_NOMEM;
         }
         F->n = n;
     }
     if (!F->st) {
-        HPCCG_HIP_TRY(hipMalloc(&F->st, sizeof(FcgState)));
+        HPCCG_HIP_TRY(dmalloc(&F->st, sizeof(FcgState)));
         HPCCG_HIP_TRY(hipHostMalloc(&F->st_h, sizeof(FcgState), hipHostMallocDefault));
     }
     if (F->hist_cap < max_iter + 2) {
-        for (auto &gg : F->graphs) hipGraphExecDestroy(gg.second);
-        F->graphs.clear();
-        hipFree(F->hist);
+        dfree(F->hist);
         F->hist = nullptr;
-        HPCCG_HIP_TRY(hipMalloc(&F->hist, 2 * (size_t)(max_iter + 2) * 8));
+        HPCCG_HIP_TRY(dmalloc(&F->hist, 2 * (size_t)(max_iter + 2) * 8));
         F->hist_cap = max_iter + 2;
     }
     FcgState init{};
@@ -1132,26 +1125,14 @@ int hpccg_fcg_learned_dev(hpccg_ctx *c, int32_t nx, int32_t ny, int32_t nz, cons
     // graphs of 1, 2, 4, ... iterations, launched in chunks that never cross a stagnation
     // check point (every `window` iterations) and are sized from the residual rate
     auto get = [&](int size, hipGraphExec_t *out) -> int {
-        GraphKey key{b, x, 200, size};
-        auto it = F->graphs.find(key);
-        if (it != F->graphs.end()) {
-            *out = it->second;
+        return cached_graph(F->graphs, c->stream, GraphKey{b, x, 200, size}, [&]() {
+            for (int i = 0; i < size; ++i) iteration();
             return 0;
-        }
-        hipGraph_t gr;
-        HPCCG_HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        for (int i = 0; i < size; ++i) iteration();
-        const hipError_t e = hipStreamEndCapture(c->stream, &gr);
-        HPCCG_HIP_TRY(e);
-        const hipError_t e2 = hipGraphInstantiate(out, gr, nullptr, nullptr, 0);
-        hipGraphDestroy(gr);
-        HPCCG_HIP_TRY(e2);
-        F->graphs[key] = *out;
-        return 0;
+        }, out);
     };
     HPCCG_TRY(capture_iteration_graphs(window, max_iter, get));
 
-    HPCCG_HIP_TRY(hipEventRecord(c->ev0, c->stream));
+    HPCCG_HIP_TRY(hipEventRecord(c->ev2, c->stream));
     // Prepare: r = b - A x, z = M(r), rz = z.r, p = z
     hipLaunchKernelGGL(kf_true<true>, dim3(Gt), dim3(kTrueThreads), 0, c->stream, sell16(A), x, b, F->r, F->st,
                        A.red_part);
@@ -1211,11 +1192,11 @@ int hpccg_fcg_learned_dev(hpccg_ctx *c, int32_t nx, int32_t ny, int32_t nz, cons
         if (st < 0) return st;
         tn = -1.0;  // the caller measures the FP64 true residual of the final x
     }
-    HPCCG_HIP_TRY(hipEventRecord(c->ev1, c->stream));
-    HPCCG_HIP_TRY(hipEventSynchronize(c->ev1));
+    HPCCG_HIP_TRY(hipEventRecord(c->ev3, c->stream));
+    HPCCG_HIP_TRY(hipEventSynchronize(c->ev3));
     if (elapsed_ms) {
         float ms = 0.f;
-        HPCCG_HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        HPCCG_HIP_TRY(hipEventElapsedTime(&ms, c->ev2, c->ev3));
         *elapsed_ms = ms;
     }
     if (fcg_iters) *fcg_iters = fcg_done;

@@ -38,6 +38,21 @@ void set_error(const char *fmt, ...);
         if (_s < 0) return _s;                                                                 \
     } while (0)
 
+// Device memory of the library's own buffers (matrix copies, workspaces, control blocks,
+// activations).  Every allocation and every free bumps one process-wide generation counter.
+// A captured graph holds the addresses of the buffers it was captured with; graph caches
+// stamp each graph with the generation at capture and replay it only while the generation is
+// unchanged, so no graph can run on a buffer that was freed or reallocated after it was made.
+// Caller-owned vectors (hpccg_dev_malloc) are part of the cache keys instead.
+uint64_t buffer_generation();
+hipError_t dmalloc_bytes(void **p, size_t bytes);
+template <typename T>
+inline hipError_t dmalloc(T **p, size_t bytes)
+{
+    return dmalloc_bytes(reinterpret_cast<void **>(p), bytes);
+}
+hipError_t dfree(void *p);
+
 constexpr int kWave = 64;
 constexpr int kNumXcd = 8;  // MI355X: 8 XCDs, each with its own L2; workgroup i -> XCD i % 8
 

@@ -123,6 +123,75 @@ struct GraphKey {
     }
 };
 
+// Instantiated graphs keyed by the caller's vectors and the launch shape, each stamped with the
+// buffer generation (common.hpp) at capture.  find() returns a graph only if no library buffer
+// has been allocated or freed since it was captured; older graphs are destroyed on the way.
+struct GraphCache {
+    struct Entry {
+        hipGraphExec_t exec;
+        uint64_t gen;
+    };
+    std::map<GraphKey, Entry> map;
+
+    hipGraphExec_t find(const GraphKey &k)
+    {
+        const uint64_t g = buffer_generation();
+        for (auto it = map.begin(); it != map.end();) {
+            if (it->second.gen != g) {
+                hipGraphExecDestroy(it->second.exec);
+                it = map.erase(it);
+            } else {
+                ++it;
+            }
+        }
+        auto it = map.find(k);
+        return it == map.end() ? nullptr : it->second.exec;
+    }
+    void put(const GraphKey &k, hipGraphExec_t e) { map[k] = Entry{e, buffer_generation()}; }
+    void clear()
+    {
+        for (auto &e : map) hipGraphExecDestroy(e.second.exec);
+        map.clear();
+    }
+};
+
+// Captures what enqueue() puts on stream s into an instantiated graph.  enqueue must not
+// allocate (a capture records no allocation, and the graph would be stale at once).
+template <typename F>
+int capture_graph(hipStream_t s, F enqueue, hipGraphExec_t *out)
+{
+    const uint64_t g0 = buffer_generation();
+    hipGraph_t g;
+    HPCCG_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const int st = enqueue();
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    if (st) {
+        if (e == hipSuccess) hipGraphDestroy(g);
+        return st;
+    }
+    HPCCG_HIP_TRY(e);
+    const hipError_t e2 = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    HPCCG_HIP_TRY(e2);
+    if (buffer_generation() != g0) {
+        hipGraphExecDestroy(*out);
+        *out = nullptr;
+        set_error("graph capture: a buffer was allocated while capturing");
+        return HPCCG_ERR_STATE;
+    }
+    return 0;
+}
+
+// The cached graph of `key`, captured by enqueue() on first use (or after any buffer change).
+template <typename F>
+int cached_graph(GraphCache &cache, hipStream_t s, const GraphKey &key, F enqueue, hipGraphExec_t *out)
+{
+    if ((*out = cache.find(key))) return 0;
+    HPCCG_TRY(capture_graph(s, enqueue, out));
+    cache.put(key, *out);
+    return 0;
+}
+
 // Workspace of the mixed-precision path (FP16 values, FP32 vectors, FP64 dots and x).
 struct MixedWs {
     int64_t n = 0;
@@ -131,7 +200,7 @@ struct MixedWs {
     void *state_h = nullptr;  // pinned host mirror
     double *hist = nullptr;   // [2][hist_cap]: recursive and true r.r per iteration
     int hist_cap = 0;
-    std::map<GraphKey, hipGraphExec_t> graphs;
+    GraphCache graphs;
 };
 
 }  // namespace hpccg
@@ -158,8 +227,9 @@ struct hpccg_ctx {
     int64_t stage_n = 0;
     double *s0 = nullptr, *s1 = nullptr, *s2 = nullptr;
 
-    std::map<hpccg::GraphKey, hipGraphExec_t> graphs;
+    hpccg::GraphCache graphs;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev2 = nullptr, ev3 = nullptr;  // an outer solve that calls cg_device_fp64 (fall-back)
 
     hpccg::MixedWs mix;
     void *ext[8] = {};  // owned by the batched / learned paths
@@ -169,7 +239,7 @@ namespace hpccg {
 void free_mixed(hpccg_ctx *c);
 void free_batched(hpccg_ctx *c);
 void free_learned(hpccg_ctx *c);
-// drop the learned path's captured graphs (they hold matrix and activation addresses)
+// drop the learned path's captured graphs (memory only: stale graphs are never replayed)
 void invalidate_learned_graphs(hpccg_ctx *c);
 int ensure_sell16(hpccg_ctx *c);
 int cg_device_fp64(hpccg_ctx *c, const double *b, double *x, int max_iter, double tol, int *niters,

@@ -7,7 +7,7 @@ extern "C" void hpccg_free_ext(hpccg_ctx *c)
     hpccg::free_batched(c);
     hpccg::free_learned(c);
     for (void *&p : c->ext) {
-        if (p) hipFree(p);
+        if (p) hpccg::dfree(p);
         p = nullptr;
     }
 }

@@ -217,17 +217,19 @@ int ensure_sell16(hpccg_ctx *c)
 {
     Matrix &A = c->A;
     if (A.sval16) return 0;
-    if ((!A.code && hipMalloc(&A.scol16, std::max<int64_t>(A.slots, 1) * sizeof(int32_t)) != hipSuccess) ||
-        hipMalloc(&A.sval16, std::max<int64_t>(A.slots, 1) * sizeof(__half)) != hipSuccess) {
-        hipFree(A.scol16);
-        hipFree(A.sval16);
+    // A non-null sval16 means a finished copy: every failure below leaves both pointers null.
+    int *flag = nullptr;
+    if (dmalloc(&flag, sizeof(int)) != hipSuccess ||
+        (!A.code && dmalloc(&A.scol16, std::max<int64_t>(A.slots, 1) * sizeof(int32_t)) != hipSuccess) ||
+        dmalloc(&A.sval16, std::max<int64_t>(A.slots, 1) * sizeof(__half)) != hipSuccess) {
+        dfree(flag);
+        dfree(A.scol16);
+        dfree(A.sval16);
         A.scol16 = nullptr;
         A.sval16 = nullptr;
         set_error("mixed precision: allocation of the FP16 sliced copy failed");
         return HPCCG_ERR_NOMEM;
     }
-    int *flag = nullptr;
-    HPCCG_HIP_TRY(hipMalloc(&flag, sizeof(int)));
     int bad = 0;
     hipError_t e = hipMemsetAsync(flag, 0, sizeof(int), c->stream);
     if (e == hipSuccess && A.nslices > 0) {
@@ -239,10 +241,10 @@ int ensure_sell16(hpccg_ctx *c)
     }
     if (e == hipSuccess) e = hipMemcpyAsync(&bad, flag, sizeof(int), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    hipFree(flag);
+    dfree(flag);
     if (e != hipSuccess || bad) {
-        hipFree(A.scol16);
-        hipFree(A.sval16);
+        dfree(A.scol16);
+        dfree(A.sval16);
         A.scol16 = nullptr;
         A.sval16 = nullptr;
         if (e != hipSuccess) {
@@ -258,14 +260,13 @@ int ensure_sell16(hpccg_ctx *c)
 void free_mixed(hpccg_ctx *c)
 {
     MixedWs &m = c->mix;
-    for (auto &g : m.graphs) hipGraphExecDestroy(g.second);
     m.graphs.clear();
-    hipFree(m.r);
-    hipFree(m.p);
-    hipFree(m.Ap);
-    hipFree(m.state);
+    dfree(m.r);
+    dfree(m.p);
+    dfree(m.Ap);
+    dfree(m.state);
     hipHostFree(m.state_h);
-    hipFree(m.hist);
+    dfree(m.hist);
     m = MixedWs{};
 }
 
@@ -273,33 +274,29 @@ static int ensure_mixed_ws(hpccg_ctx *c, int64_t n, int max_iter)
 {
     MixedWs &m = c->mix;
     if (!m.state) {
-        HPCCG_HIP_TRY(hipMalloc(&m.state, sizeof(MixState)));
+        HPCCG_HIP_TRY(dmalloc(&m.state, sizeof(MixState)));
         HPCCG_HIP_TRY(hipHostMalloc(&m.state_h, sizeof(MixState), hipHostMallocDefault));
         MixState z{};
         std::memcpy(m.state_h, &z, sizeof(z));
         HPCCG_HIP_TRY(hipMemcpy(m.state, m.state_h, sizeof(MixState), hipMemcpyHostToDevice));
     }
     if (m.n < n) {
-        for (auto &g : m.graphs) hipGraphExecDestroy(g.second);
-        m.graphs.clear();
-        hipFree(m.r);
-        hipFree(m.p);
-        hipFree(m.Ap);
+        dfree(m.r);
+        dfree(m.p);
+        dfree(m.Ap);
         m.r = m.p = m.Ap = nullptr;
         m.n = 0;
-        if (hipMalloc(&m.r, n * sizeof(float)) != hipSuccess || hipMalloc(&m.p, n * sizeof(float)) != hipSuccess ||
-            hipMalloc(&m.Ap, n * sizeof(float)) != hipSuccess) {
+        if (dmalloc(&m.r, n * sizeof(float)) != hipSuccess || dmalloc(&m.p, n * sizeof(float)) != hipSuccess ||
+            dmalloc(&m.Ap, n * sizeof(float)) != hipSuccess) {
             set_error("mixed: workspace allocation failed");
             return HPCCG_ERR_NOMEM;
         }
         m.n = n;
     }
     if (m.hist_cap < max_iter + 2) {
-        for (auto &g : m.graphs) hipGraphExecDestroy(g.second);
-        m.graphs.clear();
-        hipFree(m.hist);
+        dfree(m.hist);
         m.hist = nullptr;
-        HPCCG_HIP_TRY(hipMalloc(&m.hist, 2 * (size_t)(max_iter + 2) * sizeof(double)));
+        HPCCG_HIP_TRY(dmalloc(&m.hist, 2 * (size_t)(max_iter + 2) * sizeof(double)));
         m.hist_cap = max_iter + 2;
     }
     return 0;
@@ -348,25 +345,8 @@ static int mixed_enqueue(hpccg_ctx *c, const double *b, double *x, int count)
 
 static int mixed_graph(hpccg_ctx *c, const double *b, double *x, int chunk, hipGraphExec_t *out)
 {
-    GraphKey key{b, x, 100, chunk};
-    auto it = c->mix.graphs.find(key);
-    if (it != c->mix.graphs.end()) {
-        *out = it->second;
-        return 0;
-    }
-    hipGraph_t g;
-    HPCCG_HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    const int st = mixed_enqueue(c, b, x, chunk);
-    const hipError_t e = hipStreamEndCapture(c->stream, &g);
-    if (st) return st;
-    HPCCG_HIP_TRY(e);
-    hipGraphExec_t ge;
-    const hipError_t e2 = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-    hipGraphDestroy(g);
-    HPCCG_HIP_TRY(e2);
-    c->mix.graphs[key] = ge;
-    *out = ge;
-    return 0;
+    return cached_graph(c->mix.graphs, c->stream, GraphKey{b, x, 100, chunk},
+                        [&]() { return mixed_enqueue(c, b, x, chunk); }, out);
 }
 
 static int cg_mixed_device(hpccg_ctx *c, const double *b, double *x, int max_iter, double tol, int replace_every,
@@ -466,8 +446,8 @@ int hpccg_cg_mixed(hpccg_ctx *c, const double *b, double *x, int32_t max_iter, d
     HPCCG_HIP_TRY(hipSetDevice(c->device));
     const int64_t n = c->A.nrow;
     double *db = nullptr, *dx = nullptr;
-    if (hipMalloc(&db, n * sizeof(double)) != hipSuccess || hipMalloc(&dx, n * sizeof(double)) != hipSuccess) {
-        hipFree(db);
+    if (dmalloc(&db, n * sizeof(double)) != hipSuccess || dmalloc(&dx, n * sizeof(double)) != hipSuccess) {
+        dfree(db);
         set_error("hpccg_cg_mixed: allocation failed");
         return HPCCG_ERR_NOMEM;
     }
@@ -487,11 +467,9 @@ int hpccg_cg_mixed(hpccg_ctx *c, const double *b, double *x, int32_t max_iter, d
             st = HPCCG_ERR_HIP;
         }
     }
-    // the cached graphs captured these temporaries' addresses
-    for (auto &g : c->mix.graphs) hipGraphExecDestroy(g.second);
-    c->mix.graphs.clear();
-    hipFree(db);
-    hipFree(dx);
+    // freeing the temporaries retires the graphs captured with them (buffer generation)
+    dfree(db);
+    dfree(dx);
     return st;
 }
 

@@ -7,6 +7,12 @@ own main).  Needs the reference sources to build oracle/_ref; the fixtures it wr
 only (inputs described by grid and seed, outputs as exact float reprs and SHA-256 digests).
 
     python tests/golden/make_golden.py [--big]     # --big adds 128^3 / 256^3 cases (minutes)
+    python tests/golden/make_golden.py --exact     # only the exact-dot 128^3 history (~15 s)
+
+The exact-dot history (cg_exacthist_*.json) is written by the C restatement (oracle.cg with
+exact_dots=True: the reference's cg loop with both dots rounded once from a double-double
+accumulation), not by the reference binary: it is the yardstick for reordered (tree) dots, whose
+distance from the reference's own sequential sums grows like N*eps (1.6e-10 relative at 128^3).
 """
 from __future__ import annotations
 
@@ -167,10 +173,32 @@ def solve_case(tmp, nx, ny, nz, seed, rel_tol, max_iter):
     return rec
 
 
+def exact_hist_case(nx, ny, nz, kmax):
+    """||r_0..r_kmax|| of the cg loop with near-exact dots (oracle restatement), b = A*1."""
+    A = oracle.Problem(nx, ny, nz)
+    x, nit, _, hist, st = oracle.cg(A, max_iter=kmax, exact_dots=True)
+    assert st == 0 and nit == kmax
+    return {"source": "oracle.cg(exact_dots=True): the reference cg loop (numerics.F90:92-147) with "
+                      "both dots rounded once from a double-double sum",
+            "grid": [nx, ny, nz], "kmax": kmax, "rhs": "A*1", "hist": [float(v) for v in hist],
+            "xerr_max": float(np.max(np.abs(x - 1.0)))}
+
+
+def write_exact():
+    rec = exact_hist_case(128, 128, 128, 150)
+    with open(os.path.join(GOLD, "cg_exacthist_128x128x128_ones.json"), "w") as f:
+        json.dump(rec, f, indent=1)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--big", action="store_true")
+    ap.add_argument("--exact", action="store_true", help="only write the exact-dot 128^3 history")
     args = ap.parse_args()
+    if args.exact:
+        write_exact()
+        print("exact-dot history written to", GOLD)
+        return
     if not os.path.exists(HARNESS):
         sys.exit("oracle/_ref not built: run `make -C oracle ref` (needs the reference sources)")
     tmp = tempfile.mkdtemp(prefix="hpccg_golden_")
@@ -195,6 +223,7 @@ def main():
 
     solves = [solve_case(tmp, 32, 32, 32, 11, 1e-10, 2000), solve_case(tmp, 64, 64, 64, 12, 1e-10, 2000)]
     if args.big:
+        write_exact()
         solves += [solve_case(tmp, 128, 128, 128, 13, 1e-10, 2000), solve_case(tmp, 256, 256, 256, 14, 1e-10, 2000)]
     with open(os.path.join(GOLD, "cg_solve_sines.json"), "w") as f:
         json.dump({"source": "reference cg via oracle/_ref/serial/ref_harness solve, b = reference spmv(x*)",

@@ -97,3 +97,48 @@ def test_batch_edges(ctx):
     xd.upload(np.zeros(info["rows"]))
     res = ctx.batch_solve_dev(bd, xd, rel_tol=1e-8, max_iter=0)
     assert (res["niters"] == 0).all() and not res["converged"].any()
+
+
+def test_batch_zero_rhs_system_breaks_down_alone(ctx):
+    """b_s = 0 makes p.Ap = 0 in that system's first iteration: it stops there (x_s stays 0,
+    reported unconverged, the call returns the breakdown warning) while the other systems
+    converge as usual and the solve ends once they have."""
+    dims = np.array([[6, 5, 4], [7, 3, 5], [8, 8, 2]], np.int32)
+    info, xs, b = _batch(ctx, dims, 11)
+    off = info["row_off"]
+    b[off[1]:off[2]] = 0.0
+    bd = hpccg_amd.DeviceArray.from_numpy(ctx, b)
+    xd = hpccg_amd.DeviceArray.from_numpy(ctx, np.zeros(info["rows"]))
+    res = ctx.batch_solve_dev(bd, xd, rel_tol=1e-8, max_iter=2000)
+    assert res["status"] == hpccg_amd._lib.WARN_BREAKDOWN
+    assert res["converged"].tolist() == [1, 0, 1]
+    assert res["niters"][1] == 0
+    x = xd.numpy()
+    assert np.all(x[off[1]:off[2]] == 0.0)
+    for s in (0, 2):
+        A = oracle.Problem(*map(int, dims[s]))
+        r = b[off[s]:off[s + 1]] - oracle.spmv(A, x[off[s]:off[s + 1]])
+        assert math.sqrt(oracle.ddot1(r)) <= 1e-8 * math.sqrt(oracle.ddot1(b[off[s]:off[s + 1]]))
+        assert res["niters"][s] <= 60
+
+
+def test_batch_max_iter_2000_plans_its_launches(ctx):
+    """max_iter = 2000 (the other solvers' default): launches are sized from the residual rate,
+    so the solve costs about what the converged systems need, and a repeated solve (which
+    enqueues the previous count at once) gives the same answer bit for bit."""
+    dims = oracle.batch_dims(10, 8, 32, 31)
+    info, xs, b = _batch(ctx, dims, 3000)
+    bd = hpccg_amd.DeviceArray.from_numpy(ctx, b)
+    xd = hpccg_amd.DeviceArray.from_numpy(ctx, np.zeros(info["rows"]))
+    r1 = ctx.batch_solve_dev(bd, xd, rel_tol=1e-8, max_iter=60)
+    x1 = xd.numpy()
+    assert r1["converged"].all()
+    xd.upload(np.zeros(info["rows"]))
+    r2 = ctx.batch_solve_dev(bd, xd, rel_tol=1e-8, max_iter=2000)
+    assert r2["converged"].all() and np.array_equal(r1["niters"], r2["niters"])
+    assert np.array_equal(xd.numpy(), x1)
+    xd.upload(np.zeros(info["rows"]))
+    r3 = ctx.batch_solve_dev(bd, xd, rel_tol=1e-8, max_iter=2000)
+    assert np.array_equal(xd.numpy(), x1)
+    # 2000 captured iterations would cost ~50 us each even when idle
+    assert r3["ms"] < 20.0 + 2.0 * r1["ms"], (r1["ms"], r3["ms"])

@@ -158,16 +158,14 @@ def test_spmv_generic_csr_keeps_int32_columns(ctx):
 
 def test_upload_of_a_large_random_csr_falls_back_quickly(ctx):
     """Millions of distinct offsets: collecting them stops after a bounded number of probes per
-    row (the upload stays fast), and the copy keeps int32 columns."""
-    import time
+    row (so the upload ends; the test's own time limit is the guard), and the copy keeps int32
+    columns."""
     rng = np.random.default_rng(11)
     n, k = 300_000, 30
     cols = np.sort(rng.integers(0, n, (n, k)), axis=1).astype(np.int32).ravel() + 1
     rowindex = (1 + k * np.arange(n + 1)).astype(np.int32)
     vals = rng.standard_normal(n * k)
-    t = time.perf_counter()
     ctx.upload(rowindex, cols, vals)
-    assert time.perf_counter() - t < 20.0
     assert ctx.matrix_format()["ncodes"] == 0
     A = type("P", (), {})()  # minimal oracle.Problem stand-in for oracle.spmv
     A.n, A.rowindex, A.colindex, A.values = n, rowindex, cols, vals
@@ -180,19 +178,20 @@ def test_spmv_column_codes_banded_with_empty_rows(ctx, n):
     """At most 256 distinct offsets: one-byte column codes.  A banded CSR with 40 offsets (so
     slices are not 27 wide), rows of 0..40 entries, empty rows and a ragged last slice."""
     rng = np.random.default_rng(n)
-    offsets = np.sort(rng.choice(np.arange(-300, 301), size=40, replace=False))
+    others = np.arange(-300, 301)
+    offsets = np.sort(np.concatenate([[0], rng.choice(others[others != 0], size=39, replace=False)]))
     rows = []
     for i in range(n):
         c = i + offsets
         c = c[(c >= 0) & (c < n)]
         keep = rng.random(len(c)) < (0.0 if i % 11 == 3 else 0.8)
+        keep |= (c == i) & (i % 11 != 3)  # every row but the empty ones holds its diagonal
         rows.append(c[keep])
     lens = np.array([len(r) for r in rows])
     rowindex = np.concatenate([[1], 1 + np.cumsum(lens)]).astype(np.int32)
-    cols = (np.concatenate(rows) + 1).astype(np.int32) if lens.sum() else np.zeros(0, np.int32)
+    cols = (np.concatenate(rows) + 1).astype(np.int32)
     vals = rng.standard_normal(len(cols))
-    if len(cols) == 0:
-        pytest.skip("no entries")
+    assert len(cols) > 0
     ctx.upload(rowindex, cols, vals)
     fmt = ctx.matrix_format()
     assert 1 <= fmt["ncodes"] <= 40
@@ -276,11 +275,13 @@ def test_cg_tree_mode_within_1e12(ctx, fname):
     if A.n <= 64 ** 3:
         _, _, _, hex_, _ = oracle.cg(A, b, max_iter=g["kmax"], exact_dots=True)
     else:
-        hex_ = None
+        # the exact-dot history at 128^3 is a committed fixture (oracle.cg(exact_dots=True), 15 s on a CPU)
+        ex = load_golden("cg_exacthist_{}_ones.json".format("x".join(map(str, g["grid"]))))
+        assert ex["kmax"] == g["kmax"] and "seed" not in g
+        hex_ = np.array(ex["hist"])
     live = ref / ref[0] > 1e-10
-    if hex_ is not None:
-        dev = np.max(np.abs(hist[live] - hex_[live]) / hex_[live])
-        assert dev <= 1e-12, dev
+    dev = np.max(np.abs(hist[live] - hex_[live]) / hex_[live])
+    assert dev <= 1e-12, dev
     # against the sequential reference the gap is the reference's own summation error (~N eps)
     dev_ref = np.max(np.abs(hist[live] - ref[live]) / ref[live])
     assert dev_ref <= max(1e-12, 4 * A.n * 2 ** -53), dev_ref

@@ -119,3 +119,44 @@ def test_cg_mixed_rejects_values_not_in_fp16(ctx):
     ctx.upload(A.rowindex, A.colindex, vals)
     with pytest.raises(hpccg_amd.HpccgError):
         ctx.cg_mixed(A.b, max_iter=5)
+
+
+def test_mixed_cg_over_int32_columns(ctx):
+    """A symmetric matrix with thousands of distinct offsets col - row: the FP16 sliced copy keeps
+    int32 columns.  Values -1 off the diagonal and (row degree + 1) on it are FP16 numbers and
+    make the matrix SPD (strictly diagonally dominant).  The FP16/FP32 SpMV is the oracle's FP64
+    spmv rounded once to FP32, and the mixed solve meets 1e-10 in the oracle's FP64 true residual."""
+    rng = np.random.default_rng(17)
+    n = 5000
+    i = rng.integers(0, n, 40000)
+    j = rng.integers(0, n, 40000)
+    keep = i != j
+    pairs = set(zip(i[keep].tolist(), j[keep].tolist())) | set(zip(j[keep].tolist(), i[keep].tolist()))
+    rows = [[] for _ in range(n)]
+    for a, c in pairs:
+        rows[a].append(c)
+    cols, vals, rowindex = [], [], [1]
+    for a in range(n):
+        cs = sorted(rows[a] + [a])
+        cols += [c + 1 for c in cs]
+        vals += [float(len(cs)) if c == a else -1.0 for c in cs]
+        rowindex.append(len(cols) + 1)
+    rowindex = np.array(rowindex, np.int32)
+    cols = np.array(cols, np.int32)
+    vals = np.array(vals)
+    ctx.upload(rowindex, cols, vals)
+    assert ctx.matrix_format()["ncodes"] == 0
+    A = type("P", (), {})()  # minimal oracle.Problem stand-in for oracle.spmv
+    A.n, A.rowindex, A.colindex, A.values = n, rowindex, cols, vals
+    x32 = oracle.xstar_uniform(n, 41).astype(np.float32)
+    xd = hpccg_amd.DeviceArray.from_numpy(ctx, x32)
+    yd = hpccg_amd.DeviceArray(ctx, n, np.float32)
+    ctx.spmv_mixed_dev(xd, yd)
+    assert np.array_equal(yd.numpy(), oracle.spmv(A, x32.astype(np.float64)).astype(np.float32))
+    xs = oracle.xstar_uniform(n, 42)
+    b = oracle.spmv(A, xs)
+    normb = math.sqrt(oracle.ddot1(b))
+    x, nit, tn, hr, ht, st = ctx.cg_mixed(b, max_iter=1000, tolerance=1e-10 * normb, replace_every=25)
+    assert st == 0
+    rel = math.sqrt(oracle.ddot1(b - oracle.spmv(A, x))) / normb
+    assert rel <= 1e-10, rel

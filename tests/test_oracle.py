@@ -138,8 +138,6 @@ HIST_FILES = sorted(os.path.basename(p) for p in glob.glob(os.path.join(GOLD, "c
 def test_cg_history_bitwise_vs_reference(fname):
     g = load_golden(fname)
     nx, ny, nz = g["grid"]
-    if nx * ny * nz > 40 ** 3:
-        pytest.skip("large history checked in test_oracle_big")
     A = oracle.Problem(nx, ny, nz)
     b = A.b if "seed" not in g else oracle.spmv(A, oracle.xstar_sines(nx, ny, nz, g["seed"]))
     if "sha_b" in g:
@@ -152,11 +150,13 @@ def test_cg_history_bitwise_vs_reference(fname):
     assert normr == g["hist"][-2]  # cg returns ||r_{kmax-1}||
 
 
-def test_cg_solve_to_tolerance_vs_reference():
-    for c in load_golden("cg_solve_sines.json")["cases"]:
+@pytest.mark.parametrize("case", load_golden("cg_solve_sines.json")["cases"],
+                         ids=lambda c: "x".join(map(str, c["grid"])))
+def test_cg_solve_to_tolerance_vs_reference(case):
+    """Seeded solves to 1e-10 up to 256^3 (the 256^3 case holds ~11 GB of host memory briefly and
+    takes ~50 s): niters, normr and max|x - x*| every bit equal to the reference's."""
+    for c in [case]:
         nx, ny, nz = c["grid"]
-        if nx * ny * nz > 64 ** 3:
-            continue
         A = oracle.Problem(nx, ny, nz)
         xs = oracle.xstar_sines(nx, ny, nz, c["seed"])
         b = oracle.spmv(A, xs)
@@ -229,6 +229,27 @@ def test_planted_mistakes_are_caught(mistake):
     if mistake is None:
         _, _, _, hist_exact, _ = oracle.cg(A, max_iter=g["kmax"], exact_dots=True)
         assert hist_exact.tolist() != g["hist"]
+
+
+def test_exact_dot_history_fixture_is_the_reordered_yardstick():
+    """The committed exact-dot 128^3 history (oracle.cg with exact_dots, tests/golden/make_golden.py
+    --exact) starts at the closed-form ||b|| and departs from the reference's sequential-sum
+    history by the sequential sum's own error: about N eps / 2 relative (1.6e-10 at 128^3), not
+    more, while ||r_k|| / ||r_0|| > 1e-10; and it agrees to 5e-13 at 32^3, where N eps is small."""
+    ex = load_golden("cg_exacthist_128x128x128_ones.json")
+    ref = np.array(load_golden("cg_hist_128x128x128_ones.json")["hist"])
+    h = np.array(ex["hist"])
+    assert h[0] == ref[0] == math.sqrt(11916248.0)
+    live = ref / ref[0] > 1e-10
+    dev = np.abs(h[live] - ref[live]) / ref[live]
+    n = 128 ** 3
+    assert 0.1 * n * 2.0 ** -53 <= dev[2:].max() <= n * 2.0 ** -53, dev.max()
+    g32 = load_golden("cg_hist_32x32x32_ones.json")
+    A = oracle.Problem(32, 32, 32)
+    _, _, _, h32, _ = oracle.cg(A, max_iter=150, exact_dots=True)
+    r32 = np.array(g32["hist"])
+    live = r32 / r32[0] > 1e-10
+    assert np.max(np.abs(h32[live] - r32[live]) / r32[live]) <= 5e-13
 
 
 # ----------------------------------------------------------------------------- seeded inputs
