@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "context.hpp"
@@ -499,43 +500,73 @@ static int launch_dot(hpccg_ctx *c, int n, const double *x, const double *y, dou
     return 0;
 }
 
+// Workspace of one FP64 CG solve: the vectors r, p, Ap, the control block, the r.r history
+// (max_iter + 2 entries) and the partial sums of the SpMV (sell_grid(A) entries) and of the
+// vector kernels (kMaxGrid entries).
+struct CgWs {
+    double *r, *p, *Ap;
+    CgState *state;
+    double *rr_hist;
+    double *vec_part;
+    double *spmv_part;
+};
+static CgWs ctx_ws(hpccg_ctx *c) { return CgWs{c->r, c->p, c->Ap, c->state, c->rr_hist, c->partials, c->A.red_part}; }
+
 // Ap = A p, and in TREE mode p.Ap -> alpha, exactly as the CG loop launches it
-static void launch_cg_spmv(hpccg_ctx *c, bool tree, const double *p, double *Ap)
+static void launch_cg_spmv(hpccg_ctx *c, const CgWs &w, bool tree, const double *p, double *Ap)
 {
     const Matrix &A = c->A;
     const int G = sell_grid(A);
     if (tree) {
-        hipLaunchKernelGGL(k_cg_spmv<true>, dim3(G), dim3(kSellThreads), 0, c->stream, sell64(A), p, Ap, c->state,
-                           A.red_part);
-        hipLaunchKernelGGL(k_cg_pap_finish, dim3(1), dim3(kFinishThreads), 0, c->stream, A.red_part, G, c->state,
-                           c->rr_hist);
+        hipLaunchKernelGGL(k_cg_spmv<true>, dim3(G), dim3(kSellThreads), 0, c->stream, sell64(A), p, Ap, w.state,
+                           w.spmv_part);
+        hipLaunchKernelGGL(k_cg_pap_finish, dim3(1), dim3(kFinishThreads), 0, c->stream, w.spmv_part, G, w.state,
+                           w.rr_hist);
     } else {
-        hipLaunchKernelGGL(k_cg_spmv<false>, dim3(G), dim3(kSellThreads), 0, c->stream, sell64(A), p, Ap, c->state,
-                           A.red_part);
+        hipLaunchKernelGGL(k_cg_spmv<false>, dim3(G), dim3(kSellThreads), 0, c->stream, sell64(A), p, Ap, w.state,
+                           w.spmv_part);
     }
 }
 
 // Enqueue `count` CG iterations (each skips itself on the device once the loop has ended).
-static int enqueue_iterations(hpccg_ctx *c, double *x, int count)
+static int enqueue_iterations(hpccg_ctx *c, const CgWs &w, double *x, int count)
 {
     const int n = c->A.nrow;
     const int Gv = vec_grid(c, n);
     const bool tree = c->dot_mode == HPCCG_DOT_TREE;
     for (int it = 0; it < count; ++it) {
-        hipLaunchKernelGGL(k_cg_update_p, dim3(Gv), dim3(kVecThreads), 0, c->stream, c->r, c->p, x, n, c->state);
-        launch_cg_spmv(c, tree, c->p, c->Ap);
+        hipLaunchKernelGGL(k_cg_update_p, dim3(Gv), dim3(kVecThreads), 0, c->stream, w.r, w.p, x, n, w.state);
+        launch_cg_spmv(c, w, tree, w.p, w.Ap);
         if (tree) {
-            hipLaunchKernelGGL(k_cg_update_r<true>, dim3(Gv), dim3(kVecThreads), 0, c->stream, c->r, c->Ap, n,
-                               c->state, c->partials);
-            hipLaunchKernelGGL(k_cg_rr_finish<false>, dim3(1), dim3(kFinishThreads), 0, c->stream, c->partials, Gv,
-                               c->state, c->rr_hist);
+            hipLaunchKernelGGL(k_cg_update_r<true>, dim3(Gv), dim3(kVecThreads), 0, c->stream, w.r, w.Ap, n, w.state,
+                               w.vec_part);
+            hipLaunchKernelGGL(k_cg_rr_finish<false>, dim3(1), dim3(kFinishThreads), 0, c->stream, w.vec_part, Gv,
+                               w.state, w.rr_hist);
         } else {
-            hipLaunchKernelGGL(k_cg_serial_pap, dim3(1), dim3(kWave), 0, c->stream, c->p, c->Ap, n, c->state,
-                               c->rr_hist);
-            hipLaunchKernelGGL(k_cg_update_r<false>, dim3(Gv), dim3(kVecThreads), 0, c->stream, c->r, c->Ap, n,
-                               c->state, c->partials);
-            hipLaunchKernelGGL(k_cg_serial_rr, dim3(1), dim3(kWave), 0, c->stream, c->r, n, c->state, c->rr_hist, 0);
+            hipLaunchKernelGGL(k_cg_serial_pap, dim3(1), dim3(kWave), 0, c->stream, w.p, w.Ap, n, w.state, w.rr_hist);
+            hipLaunchKernelGGL(k_cg_update_r<false>, dim3(Gv), dim3(kVecThreads), 0, c->stream, w.r, w.Ap, n,
+                               w.state, w.vec_part);
+            hipLaunchKernelGGL(k_cg_serial_rr, dim3(1), dim3(kWave), 0, c->stream, w.r, n, w.state, w.rr_hist, 0);
         }
+    }
+    HPCCG_HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// Prepare (numerics.F90:92-99): Ap = A x; r = b - Ap; rtrans = r.r (p is formed at k = 1)
+static int enqueue_prepare(hpccg_ctx *c, const CgWs &w, const double *b, const double *x)
+{
+    const int n = c->A.nrow, Gv = vec_grid(c, n);
+    if (c->A.nslices > 0)
+        hipLaunchKernelGGL(k_spmv, dim3(sell_grid(c->A)), dim3(kSellThreads), 0, c->stream, sell64(c->A), x, w.Ap);
+    if (c->dot_mode == HPCCG_DOT_TREE) {
+        hipLaunchKernelGGL(k_cg_prep_r<true>, dim3(Gv), dim3(kVecThreads), 0, c->stream, b, w.Ap, w.r, n, w.vec_part);
+        hipLaunchKernelGGL(k_cg_rr_finish<true>, dim3(1), dim3(kFinishThreads), 0, c->stream, w.vec_part, Gv, w.state,
+                           w.rr_hist);
+    } else {
+        hipLaunchKernelGGL(k_cg_prep_r<false>, dim3(Gv), dim3(kVecThreads), 0, c->stream, b, w.Ap, w.r, n,
+                           w.vec_part);
+        hipLaunchKernelGGL(k_cg_serial_rr, dim3(1), dim3(kWave), 0, c->stream, w.r, n, w.state, w.rr_hist, 1);
     }
     HPCCG_HIP_TRY(hipGetLastError());
     return 0;
@@ -543,49 +574,39 @@ static int enqueue_iterations(hpccg_ctx *c, double *x, int count)
 
 static int get_chunk_graph(hpccg_ctx *c, const double *b, double *x, int chunk, hipGraphExec_t *out)
 {
+    const CgWs w = ctx_ws(c);
     return cached_graph(c->graphs, c->stream, GraphKey{b, x, c->dot_mode, chunk},
-                        [&]() { return enqueue_iterations(c, x, chunk); }, out);
+                        [&]() { return enqueue_iterations(c, w, x, chunk); }, out);
 }
 
-// The whole solve on device-resident b and x.
-static int cg_device(hpccg_ctx *c, const double *b, double *x, int max_iter, double tol, int *niters,
-                     double *normr, double *hist, double *elapsed_ms)
+// The solve on device-resident b and x with workspace w (allocated by the caller).  GRAPHS:
+// the iterations replay captured chunk graphs (cached per b, x); otherwise every kernel is
+// launched directly, so the solve allocates nothing at all.  *elapsed_ms: Prepare + loop +
+// final x update; *loop_ms: the iteration loop with its final x update only (the span of the
+// reference's "Total" timer, numerics.F90:106-150).
+static int cg_run(hpccg_ctx *c, const CgWs &w, bool graphs, const double *b, double *x, int max_iter, double tol,
+                  int *niters, double *normr, double *hist, double *elapsed_ms, double *loop_ms)
 {
-    HPCCG_TRY(matrix_required(c));
-    HPCCG_CHECK_ARG(max_iter >= 0, "max_iter must be >= 0");
     const int n = c->A.nrow;
-    HPCCG_TRY(ensure_workspace(c, n));
-    HPCCG_TRY(ensure_hist(c, max_iter + 2));
     const bool tree = c->dot_mode == HPCCG_DOT_TREE;
-
     CgState init{};
     init.tol = tol;
     init.k = 0;
     init.stop_at = max_iter + 1;
     init.max_iter = max_iter;
     *c->state_h = init;
-    HPCCG_HIP_TRY(hipMemcpyAsync(c->state, c->state_h, sizeof(CgState), hipMemcpyHostToDevice, c->stream));
+    HPCCG_HIP_TRY(hipMemcpyAsync(w.state, c->state_h, sizeof(CgState), hipMemcpyHostToDevice, c->stream));
 
     const int cap = tree ? 16 : 4;
     auto get = [&](int size, hipGraphExec_t *ge) { return get_chunk_graph(c, b, x, size, ge); };
-    HPCCG_TRY(capture_iteration_graphs(cap, max_iter, get));
+    if (graphs) HPCCG_TRY(capture_iteration_graphs(cap, max_iter, get));
+    auto launch = [&](int m) { return graphs ? launch_iterations(c->stream, m, get) : enqueue_iterations(c, w, x, m); };
 
     HPCCG_HIP_TRY(hipEventRecord(c->ev0, c->stream));
-    // Prepare (numerics.F90:92-99): Ap = A x; r = b - Ap; rtrans = r.r
-    HPCCG_TRY(launch_spmv(c, x, c->Ap));
-    if (tree) {
-        hipLaunchKernelGGL(k_cg_prep_r<true>, dim3(vec_grid(c, n)), dim3(kVecThreads), 0, c->stream, b, c->Ap, c->r,
-                           n, c->partials);
-        hipLaunchKernelGGL(k_cg_rr_finish<true>, dim3(1), dim3(kFinishThreads), 0, c->stream, c->partials,
-                           vec_grid(c, n), c->state, c->rr_hist);
-    } else {
-        hipLaunchKernelGGL(k_cg_prep_r<false>, dim3(vec_grid(c, n)), dim3(kVecThreads), 0, c->stream, b, c->Ap,
-                           c->r, n, c->partials);
-        hipLaunchKernelGGL(k_cg_serial_rr, dim3(1), dim3(kWave), 0, c->stream, c->r, n, c->state, c->rr_hist, 1);
-    }
-    HPCCG_HIP_TRY(hipGetLastError());
+    HPCCG_TRY(enqueue_prepare(c, w, b, x));
+    HPCCG_HIP_TRY(hipEventRecord(c->evp, c->stream));
 
-    // Iteration loop ("Total" timer, numerics.F90:106-150): graphs of up to 16 (tree) or 4
+    // Iteration loop ("Total" timer, numerics.F90:106-150): chunks of up to 16 (tree) or 4
     // (sequential dots) iterations.  With a tolerance the host reads the state after each
     // launch and sizes the next one; the reference's lagging test runs one iteration after
     // ||r|| < tol.  Without one, all max_iter iterations are enqueued at once.
@@ -593,11 +614,11 @@ static int cg_device(hpccg_ctx *c, const double *b, double *x, int max_iter, dou
     ChunkPlanner plan{cap, tol, 1};
     int launched = 0, m = std::min(cap, max_iter);
     while (launched < max_iter) {
-        HPCCG_TRY(launch_iterations(c->stream, m, get));
+        HPCCG_TRY(launch(m));
         launched += m;
         if (launched >= max_iter) break;
         if (can_stop_early) {
-            HPCCG_HIP_TRY(hipMemcpyAsync(c->state_h, c->state, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
+            HPCCG_HIP_TRY(hipMemcpyAsync(c->state_h, w.state, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
             HPCCG_HIP_TRY(hipStreamSynchronize(c->stream));
             const CgState &sh = *c->state_h;
             if (sh.k >= sh.stop_at) break;
@@ -606,28 +627,94 @@ static int cg_device(hpccg_ctx *c, const double *b, double *x, int max_iter, dou
             m = std::min(cap, max_iter - launched);
         }
     }
-    hipLaunchKernelGGL(k_cg_finish_x, dim3(vec_grid(c, n)), dim3(kVecThreads), 0, c->stream, c->p, x, n, c->state);
+    hipLaunchKernelGGL(k_cg_finish_x, dim3(vec_grid(c, n)), dim3(kVecThreads), 0, c->stream, w.p, x, n, w.state);
     HPCCG_HIP_TRY(hipGetLastError());
     HPCCG_HIP_TRY(hipEventRecord(c->ev1, c->stream));
-    HPCCG_HIP_TRY(hipMemcpyAsync(c->state_h, c->state, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
-    std::vector<double> rr(max_iter + 1);
-    HPCCG_HIP_TRY(hipMemcpyAsync(rr.data(), c->rr_hist, (max_iter + 1) * sizeof(double), hipMemcpyDeviceToHost,
-                                 c->stream));
+    HPCCG_HIP_TRY(hipMemcpyAsync(c->state_h, w.state, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
     HPCCG_HIP_TRY(hipStreamSynchronize(c->stream));
-    if (elapsed_ms) {
+    if (elapsed_ms || loop_ms) {
         float ms = 0.f;
         HPCCG_HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-        *elapsed_ms = ms;
+        if (elapsed_ms) *elapsed_ms = ms;
+        HPCCG_HIP_TRY(hipEventElapsedTime(&ms, c->evp, c->ev1));
+        if (loop_ms) *loop_ms = ms;
     }
-    const CgState &s = *c->state_h;
+    const CgState s = *c->state_h;
     const int nit = s.k - 1;
     if (niters) *niters = nit;
     // normr = ||r_{niters-1}|| (numerics.F90:122); with max_iter == 0 it stays ||r_0||
-    if (normr) *normr = std::sqrt(rr[std::max(0, nit - 1)]);
+    const int last = std::max(0, nit - 1);
+    double rr_last = 0.0;
     if (hist) {
-        for (int j = 0; j <= max_iter; ++j) hist[j] = (j <= nit) ? std::sqrt(rr[j]) : -1.0;
+        HPCCG_HIP_TRY(hipMemcpyAsync(hist, w.rr_hist, (max_iter + 1) * sizeof(double), hipMemcpyDeviceToHost,
+                                     c->stream));
+        HPCCG_HIP_TRY(hipStreamSynchronize(c->stream));
+        rr_last = hist[last];
+        for (int j = 0; j <= max_iter; ++j) hist[j] = (j <= nit) ? std::sqrt(hist[j]) : -1.0;
+    } else {
+        HPCCG_HIP_TRY(hipMemcpyAsync(c->scalar_h, w.rr_hist + last, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HPCCG_HIP_TRY(hipStreamSynchronize(c->stream));
+        rr_last = *c->scalar_h;
     }
+    if (normr) *normr = std::sqrt(rr_last);
     return s.breakdown ? HPCCG_WARN_BREAKDOWN : 0;
+}
+
+// The whole solve on device-resident b and x, with the context's workspace.
+static int cg_device(hpccg_ctx *c, const double *b, double *x, int max_iter, double tol, int *niters,
+                     double *normr, double *hist, double *elapsed_ms, double *loop_ms = nullptr)
+{
+    HPCCG_TRY(matrix_required(c));
+    HPCCG_CHECK_ARG(max_iter >= 0, "max_iter must be >= 0");
+    HPCCG_TRY(ensure_workspace(c, c->A.nrow));
+    HPCCG_TRY(ensure_hist(c, max_iter + 2));
+    return cg_run(c, ctx_ws(c), true, b, x, max_iter, tol, niters, normr, hist, elapsed_ms, loop_ms);
+}
+
+// ---- no-allocation solve ----------------------------------------------------------------------
+// The workspace of hpccg_cg_dev_noalloc lives in one file-scope device array, a copy of which
+// every device that loads this library holds until the process ends.  Layout in doubles, every
+// part 256-byte aligned: the control block (32), the r.r history (max iterations + 2), the
+// vector kernels' partials (kMaxGrid), the SpMV's partials (sell_grid of the largest matrix),
+// then r, p, Ap of HPCCG_NOALLOC_MAX_ROWS entries each: 50,561,280 bytes.  Host code never
+// names the array except to ask for its address (scratch_setup); kernels get that address.
+constexpr int64_t kNaRows = HPCCG_NOALLOC_MAX_ROWS;
+constexpr int64_t kNaOffHist = 32;
+constexpr int64_t kNaOffVec = kNaOffHist + (HPCCG_NOALLOC_MAX_ITER + 2);
+constexpr int64_t kNaOffSpmv = kNaOffVec + kMaxGrid;
+constexpr int64_t kNaSpmvPart = (kNaRows / kSlice + kSellWaves - 1) / kSellWaves;
+constexpr int64_t kNaOffR = kNaOffSpmv + kNaSpmvPart;
+constexpr int64_t kNaDoubles = kNaOffR + 3 * kNaRows;
+static_assert(sizeof(CgState) <= 32 * sizeof(double), "control block fits its slot");
+static_assert(kNaOffVec % 32 == 0 && kNaOffSpmv % 32 == 0 && kNaOffR % 32 == 0 && kNaRows % 32 == 0,
+              "256-byte aligned parts");
+
+static __device__ __align__(256) double scratch[kNaDoubles];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_dev[kMaxDevices] = {};
+
+// The device address of `device`'s copy of the workspace, looked up once per device; the
+// caller has made `device` current.
+static int scratch_setup(int device, double **out)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0 || device >= kMaxDevices) {
+        set_error("no-allocation workspace: device index %d beyond %d", device, kMaxDevices);
+        return HPCCG_ERR_ARG;
+    }
+    if (!g_scratch_dev[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error("no-allocation workspace: hipGetSymbolAddress -> %s", hipGetErrorString(e));
+            return HPCCG_ERR_HIP;
+        }
+        g_scratch_dev[device] = static_cast<double *>(p);
+    }
+    *out = g_scratch_dev[device];
+    return 0;
 }
 
 int cg_device_fp64(hpccg_ctx *c, const double *b, double *x, int max_iter, double tol, int *niters, double *normr,
@@ -649,11 +736,12 @@ int cg_trace(hpccg_ctx *c, const double *b, double *x, int max_iter, double tol,
     init.max_iter = max_iter;
     *c->state_h = init;
     HPCCG_HIP_TRY(hipMemcpyAsync(c->state, c->state_h, sizeof(CgState), hipMemcpyHostToDevice, c->stream));
-    HPCCG_TRY(launch_spmv(c, x, c->Ap));
-    hipLaunchKernelGGL(k_cg_prep_r<true>, dim3(Gv), dim3(kVecThreads), 0, c->stream, b, c->Ap, c->r, n, c->partials);
-    hipLaunchKernelGGL(k_cg_rr_finish<true>, dim3(1), dim3(kFinishThreads), 0, c->stream, c->partials, Gv, c->state,
-                       c->rr_hist);
-    HPCCG_HIP_TRY(hipGetLastError());
+    const CgWs w = ctx_ws(c);
+    const int mode = c->dot_mode;
+    c->dot_mode = HPCCG_DOT_TREE;
+    const int prep = enqueue_prepare(c, w, b, x);
+    c->dot_mode = mode;
+    HPCCG_TRY(prep);
     // p_0 = r_0 and alpha = 0 make the hook's x + alpha p equal x_0
     HPCCG_HIP_TRY(hipMemcpyAsync(c->p, c->r, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     HPCCG_HIP_TRY(hipStreamSynchronize(c->stream));
@@ -664,7 +752,7 @@ int cg_trace(hpccg_ctx *c, const double *b, double *x, int max_iter, double tol,
         if (c->state_h->k >= c->state_h->stop_at) break;
         const int k = c->state_h->k;
         hipLaunchKernelGGL(k_cg_update_p, dim3(Gv), dim3(kVecThreads), 0, c->stream, c->r, c->p, x, n, c->state);
-        launch_cg_spmv(c, true, c->p, c->Ap);
+        launch_cg_spmv(c, w, true, c->p, c->Ap);
         hipLaunchKernelGGL(k_cg_update_r<true>, dim3(Gv), dim3(kVecThreads), 0, c->stream, c->r, c->Ap, n, c->state,
                            c->partials);
         hipLaunchKernelGGL(k_cg_rr_finish<false>, dim3(1), dim3(kFinishThreads), 0, c->stream, c->partials, Gv,
@@ -716,10 +804,16 @@ int hpccg_create(hpccg_ctx **out, int device)
         dmalloc(&c->scalar, 8 * sizeof(double)) != hipSuccess ||
         hipHostMalloc(&c->scalar_h, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
-        hipEventCreate(&c->ev2) != hipSuccess || hipEventCreate(&c->ev3) != hipSuccess) {
+        hipEventCreate(&c->ev2) != hipSuccess || hipEventCreate(&c->ev3) != hipSuccess ||
+        hipEventCreate(&c->evp) != hipSuccess) {
         set_error("hpccg_create: HIP resource allocation failed");
         hpccg_destroy(c);
         return HPCCG_ERR_HIP;
+    }
+    const int sst = scratch_setup(device, &c->scratch);
+    if (sst) {
+        hpccg_destroy(c);
+        return sst;
     }
     // the control block starts zeroed (every solve rewrites it before use)
     CgState zero{};
@@ -759,6 +853,7 @@ int hpccg_destroy(hpccg_ctx *c)
     if (c->ev1) hipEventDestroy(c->ev1);
     if (c->ev2) hipEventDestroy(c->ev2);
     if (c->ev3) hipEventDestroy(c->ev3);
+    if (c->evp) hipEventDestroy(c->evp);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return 0;
@@ -1178,6 +1273,36 @@ int hpccg_cg_dev(hpccg_ctx *c, const double *b, double *x, int32_t max_iter, dou
     HPCCG_CHECK_ARG(c && b && x, "null argument");
     HPCCG_HIP_TRY(hipSetDevice(c->device));
     return cg_device(c, b, x, max_iter, tolerance, niters, normr, hist, elapsed_ms);
+}
+
+// No reference counterpart: hpccg_cg_dev for callers that cannot allocate device memory.  The
+// workspace is this device's static array (scratch_setup); the kernels, their order and the
+// result are hpccg_cg_dev's, but every kernel is launched directly (no graph is captured or
+// instantiated) and nothing is allocated.
+int hpccg_cg_dev_noalloc(hpccg_ctx *c, const double *b, double *x, int32_t max_iter, double tolerance,
+                         int32_t *niters, double *normr, double *hist, double *elapsed_ms)
+{
+    HPCCG_CHECK_ARG(c && b && x, "null argument");
+    HPCCG_TRY(matrix_required(c));
+    if (c->A.nrow > HPCCG_NOALLOC_MAX_ROWS) {
+        set_error("hpccg_cg_dev_noalloc: %d rows exceed the static workspace (at most %d rows)", c->A.nrow,
+                  (int)HPCCG_NOALLOC_MAX_ROWS);
+        return HPCCG_ERR_ARG;
+    }
+    if (max_iter < 0 || max_iter > HPCCG_NOALLOC_MAX_ITER) {
+        set_error("hpccg_cg_dev_noalloc: max_iter %d outside 0..%d (static workspace)", max_iter,
+                  (int)HPCCG_NOALLOC_MAX_ITER);
+        return HPCCG_ERR_ARG;
+    }
+    if (!c->scratch) {
+        set_error("hpccg_cg_dev_noalloc: no workspace address for device %d", c->device);
+        return HPCCG_ERR_STATE;
+    }
+    HPCCG_HIP_TRY(hipSetDevice(c->device));
+    double *s = c->scratch;
+    const CgWs w{s + kNaOffR, s + kNaOffR + kNaRows, s + kNaOffR + 2 * kNaRows, reinterpret_cast<CgState *>(s),
+                 s + kNaOffHist, s + kNaOffVec, s + kNaOffSpmv};
+    return cg_run(c, w, false, b, x, max_iter, tolerance, niters, normr, hist, elapsed_ms, nullptr);
 }
 
 // ---- host-array drop-ins ---------------------------------------------------------------------

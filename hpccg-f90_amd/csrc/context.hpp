@@ -230,6 +230,8 @@ struct hpccg_ctx {
     hpccg::GraphCache graphs;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t ev2 = nullptr, ev3 = nullptr;  // an outer solve that calls cg_device_fp64 (fall-back)
+    hipEvent_t evp = nullptr;                 // end of the FP64 CG's Prepare
+    double *scratch = nullptr;  // this device's static workspace of the no-allocation solve
 
     hpccg::MixedWs mix;
     void *ext[8] = {};  // owned by the batched / learned paths

@@ -13,6 +13,8 @@ HEADER = os.path.join(os.path.dirname(PKG_ROOT), "include", "hpccg_hip.h")
 DOT_SEQUENTIAL = 0
 DOT_TREE = 1
 WARN_BREAKDOWN = 1
+NOALLOC_MAX_ROWS = 2097152  # include/hpccg_hip.h: static workspace of hpccg_cg_dev_noalloc
+NOALLOC_MAX_ITER = 4094
 
 P = ctypes.c_void_p
 I32 = ctypes.c_int32
@@ -56,6 +58,7 @@ _SIGS = {
     "hpccg_wwpby_dev": (ctypes.c_int, [P, I32, D, P, P]),
     "hpccg_wxbw_dev": (ctypes.c_int, [P, I32, P, D, P]),
     "hpccg_cg_dev": (ctypes.c_int, [P, P, P, I32, D, PI32, PD, P, PD]),
+    "hpccg_cg_dev_noalloc": (ctypes.c_int, [P, P, P, I32, D, PI32, PD, P, PD]),
     "hpccg_cg_mixed": (ctypes.c_int, [P, P, P, I32, D, I32, PI32, PD, P, P]),
     "hpccg_cg_mixed_dev": (ctypes.c_int, [P, P, P, I32, D, I32, PI32, PD, P, P, PD]),
     "hpccg_spmv_mixed_dev": (ctypes.c_int, [P, P, P]),
@@ -333,6 +336,15 @@ class Context:
         st = _check(lib().hpccg_cg_dev(self.h, b.ptr, x.ptr, max_iter, tolerance, ctypes.byref(nit),
                                        ctypes.byref(nr), _ptr(hist) if want_hist else None,
                                        ctypes.byref(ms)), "cg_dev")
+        return {"niters": nit.value, "normr": nr.value, "hist": hist, "status": st, "ms": ms.value}
+
+    def cg_dev_noalloc(self, b: DeviceArray, x: DeviceArray, max_iter=150, tolerance=0.0, want_hist=True):
+        """hpccg_cg_dev on the device's static workspace: allocates nothing (limits in the header)."""
+        hist = np.zeros(max_iter + 1) if want_hist else None
+        nit, nr, ms = ctypes.c_int32(), ctypes.c_double(), ctypes.c_double()
+        st = _check(lib().hpccg_cg_dev_noalloc(self.h, b.ptr, x.ptr, max_iter, tolerance, ctypes.byref(nit),
+                                               ctypes.byref(nr), _ptr(hist) if want_hist else None,
+                                               ctypes.byref(ms)), "cg_dev_noalloc")
         return {"niters": nit.value, "normr": nr.value, "hist": hist, "status": st, "ms": ms.value}
 
     def cg_mixed(self, b, x0=None, max_iter=1000, tolerance=0.0, replace_every=25):

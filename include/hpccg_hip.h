@@ -130,6 +130,18 @@ int hpccg_compute_residual_dev(hpccg_ctx *ctx, int32_t n, const double *v1, cons
 int hpccg_cg_dev(hpccg_ctx *ctx, const double *b, double *x, int32_t max_iter, double tolerance,
                  int32_t *niters, double *normr, double *hist, double *elapsed_ms);
 
+/* No counterpart in the reference.  hpccg_cg_dev for callers that cannot allocate device memory
+ * (serving loops, harnesses that forbid allocation): same kernels in the same order, so the same
+ * results bit for bit, but the workspace (r, p, Ap, control block, residual history, partial
+ * sums) is a static array that each device holds for the life of the process (50.6 MB), its
+ * kernels are launched directly (no graph is captured), and the call allocates nothing.  Limits:
+ * nrow <= HPCCG_NOALLOC_MAX_ROWS (128^3) and 0 <= max_iter <= HPCCG_NOALLOC_MAX_ITER; larger
+ * sizes return HPCCG_ERR_ARG.  Every call on one device shares that device's workspace: calls
+ * that overlap on one device (two contexts or threads) race.  Not reentrant. */
+enum { HPCCG_NOALLOC_MAX_ROWS = 2097152, HPCCG_NOALLOC_MAX_ITER = 4094 };
+int hpccg_cg_dev_noalloc(hpccg_ctx *ctx, const double *b, double *x, int32_t max_iter, double tolerance,
+                         int32_t *niters, double *normr, double *hist, double *elapsed_ms);
+
 /* ---- mixed precision (no reference counterpart: extends cg, numerics.F90:41-162) ----------
  * FP16 matrix values (exact for generate_matrix's 27 / -1; other matrices are rejected unless
  * every value is an FP16 number), int32 columns, FP32 r/p/Ap with FP64 arithmetic inside the
