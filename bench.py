@@ -18,6 +18,7 @@ import argparse
 import json
 import math
 import os
+import shutil
 import subprocess
 import sys
 import tempfile
@@ -41,73 +42,186 @@ def spmv_bytes(n, nnz, s_val=8, s_idx=4, s_x=8, s_y=8):
     return nnz * (s_val + s_idx) + (n + 1) * 4 + n * s_x + n * s_y
 
 
+def host_description():
+    """CPU facts of this host for the baseline record: model, sockets, NUMA nodes, logical CPUs,
+    the CPUs this process may run on, its physical cores and the cgroup CPU quota."""
+    info = {"nproc": os.cpu_count()}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=30).stdout
+        for ln in out.splitlines():
+            k, _, v = ln.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "Model name":
+                info["model"] = v
+            elif k == "Socket(s)":
+                info["sockets"] = int(v)
+            elif k == "NUMA node(s)":
+                info["numa_nodes"] = int(v)
+            elif k == "Thread(s) per core":
+                info["threads_per_core"] = int(v)
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    cpus = sorted(os.sched_getaffinity(0))
+    info["affinity_cpus"] = len(cpus)
+    cores = set()
+    for c in cpus:
+        try:
+            with open(f"/sys/devices/system/cpu/cpu{c}/topology/core_id") as f:
+                core = f.read().strip()
+            with open(f"/sys/devices/system/cpu/cpu{c}/topology/physical_package_id") as f:
+                pkg = f.read().strip()
+            cores.add((pkg, core))
+        except OSError:
+            cores.add(("?", str(c)))
+    info["physical_cores"] = len(cores)
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            quota, period = f.read().split()
+        if quota != "max":
+            info["cgroup_cpus"] = round(int(quota) / int(period), 2)
+    except (OSError, ValueError):
+        pass
+    return info
+
+
+def cpu_threads(host):
+    """Threads for the reference's OpenMP build: the host's own OMP_NUM_THREADS when it sets one
+    (the CPU share this GPU's slot is given), else the physical cores this process may use,
+    capped by the cgroup quota."""
+    env = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    if env > 0:
+        return env, "OMP_NUM_THREADS set by the host (this GPU's CPU share)"
+    n = host.get("physical_cores") or os.cpu_count() or 1
+    if host.get("cgroup_cpus"):
+        n = max(1, min(n, int(host["cgroup_cpus"])))
+    return n, "physical cores available to the process"
+
+
+def run_reference(exe, args, threads, host, timeout=600):
+    """One run of a reference binary under the OpenMP placement of the baseline plan: threads
+    spread over cores, and memory interleaved over NUMA nodes on multi-node hosts (the reference
+    has no first-touch initialisation, SparseMatrix.F90:122-136)."""
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="spread", OMP_PLACES="cores")
+    cmd = [exe] + [str(a) for a in args]
+    numa = (host.get("numa_nodes") or 1) > 1 and shutil.which("numactl")
+    if numa:
+        cmd = ["numactl", "--interleave=all"] + cmd
+    t0 = time.perf_counter()
+    p = subprocess.run(cmd, capture_output=True, text=True, env=env, timeout=timeout)
+    wall = time.perf_counter() - t0
+    if p.returncode != 0:
+        log("reference run failed:", " ".join(cmd), p.stderr[-2000:])
+        return None, wall, bool(numa)
+    return p.stdout, wall, bool(numa)
+
+
+def timer_value(stdout, region):
+    for ln in stdout.splitlines():
+        if "TIMER:" in ln and region in ln.split():
+            return float(ln.split()[-2])
+    return None
+
+
 def cpu_baseline(grid, b, gpu_niters, log_dir):
     """The reference's own OpenMP build (oracle/_ref/omp, compiled from MPI/src) solving the same
-    system on this host; its "TIMER: Total" (the iteration loop) is the reported value."""
+    256^3 system on this host (R4 of the baseline plan): its "TIMER: Total" (the iteration loop)
+    is the reported value.  Also the plan's fixed-iteration runs: R1 = 150 iterations at 32^3
+    and 128^3 (OpenMP), R2 = the SpMV rate at 128^3 from the TIMINGLEV=3 build's MFLOPS line,
+    R3 = the serial build on one core at 32^3."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle  # noqa: F401  (locates the reference build)
-    exe = oracle.ref_binary("omp", "ref_harness")
-    if exe is None:
+    harness = oracle.ref_binary("omp", "ref_harness")
+    if harness is None:
         return None
-    cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    host = host_description()
+    threads, why = cpu_threads(host)
     nx, ny, nz = grid
     normb = math.sqrt(float(b @ b))
     with tempfile.TemporaryDirectory() as td:
         fb, fx = os.path.join(td, "b.bin"), os.path.join(td, "x.bin")
         b.tofile(fb)
-        env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="spread", OMP_PLACES="cores")
-        t0 = time.perf_counter()
-        p = subprocess.run([exe, "solve", str(nx), str(ny), str(nz), "2000", repr(REL_TOL * normb), fb, fx],
-                           capture_output=True, text=True, env=env, timeout=600)
-        wall = time.perf_counter() - t0
-    if p.returncode != 0:
-        log("cpu baseline failed:", p.stderr[-2000:])
+        out, wall, numa = run_reference(harness, ["solve", nx, ny, nz, 2000, repr(REL_TOL * normb), fb, fx],
+                                        threads, host)
+    if out is None:
         return None
-    total = None
+    total = timer_value(out, "Total")
     niters = None
-    for ln in p.stdout.splitlines():
-        if "TIMER:" in ln and "Total" in ln:
-            total = float(ln.split("Total")[1].split()[0])
+    for ln in out.splitlines():
         if ln.strip().startswith("H SOLVE"):
             niters = int(ln.split()[2])
     with open(os.path.join(log_dir, "cpu_baseline_stdout.txt"), "w") as f:
-        f.write(p.stdout)
-    return {"value": total, "unit": "s", "cores": cores, "kind": "reference",
+        f.write(out)
+    plan = {}
+    omp, omp3, ser = (oracle.ref_binary(v, "HPCCG") for v in ("omp", "omp3", "serial"))
+    for n in (32, 128):
+        o, _, _ = run_reference(omp, [n, n, n], threads, host) if omp else (None, 0, 0)
+        t = timer_value(o, "Total") if o else None
+        plan[f"R1_omp_150it_{n}^3_s"] = t
+    if omp3:
+        o, _, _ = run_reference(omp3, [128, 128, 128], threads, host)
+        if o:
+            mflops = float([ln for ln in o.splitlines() if "MFLOPS SPARSEMV" in ln][0].split()[-1])
+            N, nnz = 128 ** 3, (3 * 128 - 2) ** 3
+            t_spmv = 2.0 * 27 * N / (mflops * 1e6)  # the reference's nominal flop count per SpMV
+            plan["R2_omp_128^3_mflops_sparsemv"] = mflops
+            plan["R2_omp_128^3_spmv_GBs"] = round(spmv_bytes(N, nnz) / t_spmv / 1e9, 1)
+    if ser:
+        o, _, _ = run_reference(ser, [32, 32, 32], 1, host)
+        plan["R3_serial_1core_150it_32^3_s"] = timer_value(o, "Total") if o else None
+    return {"value": total, "unit": "s", "cores": threads, "kind": "reference",
             "sample": f"full solve of the same {nx}x{ny}x{nz} system to 1e-10 by the reference cg "
-                      f"(MPI/src, amdflang -O3 -fopenmp -D_OMP), its TIMER: Total; {niters} iterations "
-                      f"(GPU FP64 CG: {gpu_niters}); process wall incl. generate_matrix {wall:.1f} s"}
+                      f"(MPI/src, amdflang -O3 -fopenmp -D_OMP), its TIMER: Total (the iteration loop); "
+                      f"{niters} iterations (GPU FP64 CG: {gpu_niters}); process wall incl. "
+                      f"generate_matrix {wall:.1f} s",
+            "threads_from": why, "placement": "OMP_PROC_BIND=spread OMP_PLACES=cores"
+                                              + (" numactl --interleave=all" if numa else ""),
+            "host": host, "plan": plan}
 
 
-def measure_roofline(ctx, xs, n, nnz, n1, reps):
-    """Roofline of the dominant kernel: the FP64 CSR SpMV as the CG loop launches it, timed with
-    HIP events on the context's stream, plus a plain copy of similar size as the HBM calibration."""
+def measure_roofline(ctx, xs, n, nnz, n1, reps, b_h=None, tol=None):
+    """Roofline of the dominant kernel: the FP64 SpMV as the CG loop launches it, timed with HIP
+    events on the context's stream, against the bytes its device format must move (`frac`); the
+    CSR formula's bytes give `effective_frac`.  With b_h, also the same kernel inside a CG solve
+    (one iteration at a time, each SpMV between HIP events, p rewritten just before it)."""
+    import numpy as np
     import hpccg_amd
     y = hpccg_amd.DeviceArray(ctx, n)
     spmv_ms = ctx.time_spmv(xs, y, reps=reps)
     B = spmv_bytes(n, nnz)
-    achieved = B / (spmv_ms * 1e-3) / 1e9
-    traffic = None
+    # the device copy stores one-byte column codes instead of int32 columns when the matrix has
+    # few distinct offsets col - row: 8 + 1 bytes per entry, the row pointers, x once and y
+    fmt = ctx.matrix_format()
+    Bf = spmv_bytes(n, nnz, s_idx=1 if fmt["ncodes"] else 4)
+    achieved = Bf / (spmv_ms * 1e-3) / 1e9
+    traffic, tsrc = None, None
     tf = os.path.join(ROOT, "profiles", f"spmv_fp64_{n1}_traffic.json")
     if os.path.exists(tf):
         with open(tf) as f:
-            traffic = json.load(f).get("traffic_bytes")
-    # the device copy stores one-byte column codes instead of int32 columns when the matrix has
-    # few distinct offsets col - row: the bytes that format must move, and their rate
-    fmt = ctx.matrix_format()
-    Bf = spmv_bytes(n, nnz, s_idx=1 if fmt["ncodes"] else 4)
+            t = json.load(f)
+        traffic = t.get("traffic_bytes")
+        tsrc = {k: t.get(k) for k in ("measured_at", "fetch_scale", "write_scale", "source")}
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "k_cg_spmv<true> (FP64 sliced-ELL SpMV of the CSR + p.Ap partials)",
-                "algorithmic_bytes": B, "avg_ms": round(spmv_ms, 5),
-                "column_codes": fmt["ncodes"], "format_bytes": Bf,
-                "format_GBs": round(Bf / (spmv_ms * 1e-3) / 1e9, 1),
-                "format_frac": round(Bf / (spmv_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+                "kernel": "k_cg_spmv<true> (FP64 sliced-ELL SpMV with one-byte column codes + p.Ap partials)",
+                "algorithmic_bytes": Bf, "bytes_per_entry": 9 if fmt["ncodes"] else 12,
+                "avg_ms": round(spmv_ms, 5), "column_codes": fmt["ncodes"],
+                "traffic_source": tsrc,
+                "traffic_over_bytes": round(traffic / Bf, 4) if traffic else None,
+                "csr_formula_bytes": B, "effective_GBs": round(B / (spmv_ms * 1e-3) / 1e9, 1),
+                "effective_frac": round(B / (spmv_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     cb = min(B // 2, 2 ** 31) // 16 * 16
     ca = hpccg_amd.DeviceArray(ctx, cb // 8)
     cbuf = hpccg_amd.DeviceArray(ctx, cb // 8)
     copy_ms = ctx.time_copy(cbuf, ca, reps=20)
     roofline["copy_GBs"] = round(2 * cb / (copy_ms * 1e-3) / 1e9, 1)
-    log(f"spmv {spmv_ms:.4f} ms -> {achieved:.0f} GB/s ({100 * achieved / HBM_PEAK_GBS:.1f}% of 8 TB/s); "
+    if b_h is not None:
+        r = ctx.cg_timed(b_h, max_iter=2000, tolerance=tol, level=2)
+        ts = np.asarray(r["t_spmv"][1:]) * 1e3  # iteration 1 reads p = r written by Prepare's chain
+        roofline["inloop_avg_ms"] = round(float(ts.mean()), 5)
+        roofline["inloop_frac"] = round(Bf / (ts.mean() * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+        roofline["inloop_launches"] = int(len(ts))
+    log(f"spmv {spmv_ms:.4f} ms -> {achieved:.0f} GB/s of format bytes ({100 * achieved / HBM_PEAK_GBS:.1f}% of "
+        f"8 TB/s; CSR formula {roofline['effective_frac']:.3f}); in-loop {roofline.get('inloop_avg_ms')} ms; "
         f"copy {roofline['copy_GBs']} GB/s")
     return roofline
 
@@ -182,8 +296,12 @@ def main():
         print(json.dumps({"network": measure_network(ctx, grid, args.spmv_reps)}), flush=True)
         ctx.close()
         return
+    # matrix setup, reported apart from the solves: the CSR built on the device
+    # (generate_matrix) plus its FP64 coded sliced copy, then the FP16 copy the mixed and
+    # flexible solvers read (built on their first use)
     t0 = time.perf_counter()
     ctx.generate(*grid)
+    setup = {"generate_and_fp64_copy_s": round(time.perf_counter() - t0, 4)}
     info = ctx.matrix_info()
     n, nnz = info["nrow"], info["nnz"]
     xs_h = hpccg_amd.xstar_sines(*grid, SEED)
@@ -195,7 +313,14 @@ def main():
     x = hpccg_amd.DeviceArray(ctx, n)
     r = hpccg_amd.DeviceArray(ctx, n)
     zeros = np.zeros(n)
-    log(f"setup {time.perf_counter() - t0:.1f} s: grid {grid}, n={n}, nnz={nnz}, ||b||={normb:.6e}")
+    if not args.spmv_only:
+        x32 = hpccg_amd.DeviceArray(ctx, n, np.float32)
+        y32 = hpccg_amd.DeviceArray(ctx, n, np.float32)
+        t1 = time.perf_counter()
+        ctx.spmv_mixed_dev(x32, y32)
+        setup["fp16_copy_s"] = round(time.perf_counter() - t1, 4)
+        del x32, y32
+    log(f"setup {setup}: grid {grid}, n={n}, nnz={nnz}, ||b||={normb:.6e}")
     if args.spmv_only:
         print(json.dumps({"roofline": measure_roofline(ctx, xs, n, nnz, n1, args.spmv_reps)}), flush=True)
         ctx.close()
@@ -261,11 +386,11 @@ def main():
     for k in results:
         results[k]["speedup_vs_fp64_cg"] = round(results["fp64_cg"]["s_wall"] / results[k]["s_wall"], 3)
 
-    roofline = measure_roofline(ctx, xs, n, nnz, n1, args.spmv_reps)
+    roofline = measure_roofline(ctx, xs, n, nnz, n1, args.spmv_reps, b_h=b_h, tol=REL_TOL * normb)
 
-    # batched solves (64 systems, extents ~ U{16..96}, x* ~ U(-1,1), to 1e-8 each, one graph launch
-    # of at most 100 iterations: the systems need 37-43, and every captured iteration past the
-    # last system's convergence still costs its seven launches, about 50 us)
+    # batched solves (64 systems, extents ~ U{16..96}, x* ~ U(-1,1), to 1e-8 each; the solve
+    # enqueues graph chunks sized from the batch's residual rate, and a repeated solve the count
+    # of the previous one at once)
     batched = None
     if not args.no_batched:
         dims = hpccg_amd.batch_dims(64, 16, 96, 2024)
@@ -280,17 +405,18 @@ def main():
         t_tune = time.perf_counter()
         best, var_ms = ctx.batch_autotune(bb, xbd, probe_iters=8)
         t_tune = time.perf_counter() - t_tune
-        ctx.batch_solve_dev(bb, xbd, rel_tol=1e-8, max_iter=100)  # capture + warm
+        ctx.batch_solve_dev(bb, xbd, rel_tol=1e-8, max_iter=2000)  # capture + warm
         xbd.upload(np.zeros(binfo["rows"]))
         t = time.perf_counter()
-        res = ctx.batch_solve_dev(bb, xbd, rel_tol=1e-8, max_iter=100)
+        res = ctx.batch_solve_dev(bb, xbd, rel_tol=1e-8, max_iter=2000)
         t = time.perf_counter() - t
         variants = ctx.batch_variants()
         batched = {"systems": 64, "rows": binfo["rows"], "nnz": binfo["nnz"], "s_wall": round(t, 6),
                    "s_device": round(res["ms"] / 1e3, 6), "converged": int(res["converged"].sum()),
                    "max_relres": float(res["relres"].max()), "iters_min_max": [int(res["niters"].min()),
                                                                                int(res["niters"].max())],
-                   "autotune_s": round(t_tune, 3), "variant": {"rows_per_wave": variants[best][0], "waves_per_wg": variants[best][1]},
+                   "max_iter": 2000, "autotune_s": round(t_tune, 3),
+                   "variant": {"rows_per_wave": variants[best][0], "waves_per_wg": variants[best][1]},
                    "variant_ms": [round(v, 4) for v in var_ms.tolist()]}
         log(f"batched: {batched}")
 
@@ -317,6 +443,7 @@ def main():
                    "solver": headline, "rows": n, "nnz": nnz},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "setup": setup,
         "solvers": results,
         "fp64_cg_host_arrays": host_arrays,
         "learned_training": train_info,

@@ -11,6 +11,7 @@
 //     HPCCG_DOT_TREE uses a fixed wave/workgroup/grid tree (fused multiply-adds inside).
 
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <algorithm>
 #include <atomic>
@@ -585,7 +586,8 @@ static int get_chunk_graph(hpccg_ctx *c, const double *b, double *x, int chunk, 
 // final x update; *loop_ms: the iteration loop with its final x update only (the span of the
 // reference's "Total" timer, numerics.F90:106-150).
 static int cg_run(hpccg_ctx *c, const CgWs &w, bool graphs, const double *b, double *x, int max_iter, double tol,
-                  int *niters, double *normr, double *hist, double *elapsed_ms, double *loop_ms)
+                  int *niters, double *normr, double *hist, double *elapsed_ms, double *loop_ms,
+                  bool ranges = false)
 {
     const int n = c->A.nrow;
     const bool tree = c->dot_mode == HPCCG_DOT_TREE;
@@ -602,9 +604,17 @@ static int cg_run(hpccg_ctx *c, const CgWs &w, bool graphs, const double *b, dou
     if (graphs) HPCCG_TRY(capture_iteration_graphs(cap, max_iter, get));
     auto launch = [&](int m) { return graphs ? launch_iterations(c->stream, m, get) : enqueue_iterations(c, w, x, m); };
 
+    if (ranges) roctxRangePushA("Prepare");
     HPCCG_HIP_TRY(hipEventRecord(c->ev0, c->stream));
     HPCCG_TRY(enqueue_prepare(c, w, b, x));
     HPCCG_HIP_TRY(hipEventRecord(c->evp, c->stream));
+    if (ranges) {
+        // the host ranges bracket the device work: wait for Prepare before closing its range
+        const hipError_t e = hipEventSynchronize(c->evp);
+        roctxRangePop();
+        roctxRangePushA("Total");
+        HPCCG_HIP_TRY(e);
+    }
 
     // Iteration loop ("Total" timer, numerics.F90:106-150): chunks of up to 16 (tree) or 4
     // (sequential dots) iterations.  With a tolerance the host reads the state after each
@@ -631,7 +641,9 @@ static int cg_run(hpccg_ctx *c, const CgWs &w, bool graphs, const double *b, dou
     HPCCG_HIP_TRY(hipGetLastError());
     HPCCG_HIP_TRY(hipEventRecord(c->ev1, c->stream));
     HPCCG_HIP_TRY(hipMemcpyAsync(c->state_h, w.state, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
-    HPCCG_HIP_TRY(hipStreamSynchronize(c->stream));
+    const hipError_t esync = hipStreamSynchronize(c->stream);
+    if (ranges) roctxRangePop();
+    HPCCG_HIP_TRY(esync);
     if (elapsed_ms || loop_ms) {
         float ms = 0.f;
         HPCCG_HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
@@ -657,6 +669,96 @@ static int cg_run(hpccg_ctx *c, const CgWs &w, bool graphs, const double *b, dou
         rr_last = *c->scalar_h;
     }
     if (normr) *normr = std::sqrt(rr_last);
+    return s.breakdown ? HPCCG_WARN_BREAKDOWN : 0;
+}
+
+// cg with the reference's timer regions (numerics.F90:91-156), iterations launched one at a
+// time: HIP events bracket Prepare, every iteration and its SpMV, roctx ranges of the same
+// names bracket the same device work (the host waits at the end of each), and the host tests
+// the stopping rule before each iteration as the reference does (numerics.F90:109).  Same
+// kernels in the same order as cg_run, so the same results bit for bit.  Times in seconds;
+// t_iter / t_spmv hold max_iter entries (k-th iteration at [k-1]).
+static int cg_timed_iterations(hpccg_ctx *c, const double *b, double *x, int max_iter, double tol, int *niters,
+                               double *normr, double *hist, double *t_total, double *t_prepare, double *t_iter,
+                               double *t_spmv)
+{
+    const int n = c->A.nrow, Gv = vec_grid(c, n), G = sell_grid(c->A);
+    const bool tree = c->dot_mode == HPCCG_DOT_TREE;
+    const CgWs w = ctx_ws(c);
+    CgState init{};
+    init.tol = tol;
+    init.stop_at = max_iter + 1;
+    init.max_iter = max_iter;
+    *c->state_h = init;
+    HPCCG_HIP_TRY(hipMemcpyAsync(w.state, c->state_h, sizeof(CgState), hipMemcpyHostToDevice, c->stream));
+    float ms = 0.f;
+    roctxRangePushA("Prepare");
+    HPCCG_HIP_TRY(hipEventRecord(c->ev0, c->stream));
+    HPCCG_TRY(enqueue_prepare(c, w, b, x));
+    HPCCG_HIP_TRY(hipEventRecord(c->evp, c->stream));
+    HPCCG_HIP_TRY(hipMemcpyAsync(c->state_h, w.state, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
+    hipError_t e = hipStreamSynchronize(c->stream);
+    roctxRangePop();
+    HPCCG_HIP_TRY(e);
+    HPCCG_HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->evp));
+    if (t_prepare) *t_prepare = ms * 1e-3;
+    roctxRangePushA("Total");
+    for (int k = 1; k <= max_iter && c->state_h->k < c->state_h->stop_at; ++k) {
+        roctxRangePushA("Iteration");
+        HPCCG_HIP_TRY(hipEventRecord(c->ev2, c->stream));
+        hipLaunchKernelGGL(k_cg_update_p, dim3(Gv), dim3(kVecThreads), 0, c->stream, w.r, w.p, x, n, w.state);
+        roctxRangePushA("spmv");
+        HPCCG_HIP_TRY(hipEventRecord(c->evs0, c->stream));
+        if (tree)
+            hipLaunchKernelGGL(k_cg_spmv<true>, dim3(G), dim3(kSellThreads), 0, c->stream, sell64(c->A), w.p, w.Ap,
+                               w.state, w.spmv_part);
+        else
+            hipLaunchKernelGGL(k_cg_spmv<false>, dim3(G), dim3(kSellThreads), 0, c->stream, sell64(c->A), w.p, w.Ap,
+                               w.state, w.spmv_part);
+        HPCCG_HIP_TRY(hipEventRecord(c->evs1, c->stream));
+        e = hipEventSynchronize(c->evs1);
+        roctxRangePop();
+        HPCCG_HIP_TRY(e);
+        if (tree) {
+            hipLaunchKernelGGL(k_cg_pap_finish, dim3(1), dim3(kFinishThreads), 0, c->stream, w.spmv_part, G, w.state,
+                               w.rr_hist);
+            hipLaunchKernelGGL(k_cg_update_r<true>, dim3(Gv), dim3(kVecThreads), 0, c->stream, w.r, w.Ap, n, w.state,
+                               w.vec_part);
+            hipLaunchKernelGGL(k_cg_rr_finish<false>, dim3(1), dim3(kFinishThreads), 0, c->stream, w.vec_part, Gv,
+                               w.state, w.rr_hist);
+        } else {
+            hipLaunchKernelGGL(k_cg_serial_pap, dim3(1), dim3(kWave), 0, c->stream, w.p, w.Ap, n, w.state, w.rr_hist);
+            hipLaunchKernelGGL(k_cg_update_r<false>, dim3(Gv), dim3(kVecThreads), 0, c->stream, w.r, w.Ap, n,
+                               w.state, w.vec_part);
+            hipLaunchKernelGGL(k_cg_serial_rr, dim3(1), dim3(kWave), 0, c->stream, w.r, n, w.state, w.rr_hist, 0);
+        }
+        HPCCG_HIP_TRY(hipGetLastError());
+        HPCCG_HIP_TRY(hipEventRecord(c->ev3, c->stream));
+        HPCCG_HIP_TRY(hipMemcpyAsync(c->state_h, w.state, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
+        e = hipStreamSynchronize(c->stream);
+        roctxRangePop();
+        HPCCG_HIP_TRY(e);
+        HPCCG_HIP_TRY(hipEventElapsedTime(&ms, c->ev2, c->ev3));
+        if (t_iter) t_iter[k - 1] = ms * 1e-3;
+        HPCCG_HIP_TRY(hipEventElapsedTime(&ms, c->evs0, c->evs1));
+        if (t_spmv) t_spmv[k - 1] = ms * 1e-3;
+    }
+    hipLaunchKernelGGL(k_cg_finish_x, dim3(Gv), dim3(kVecThreads), 0, c->stream, w.p, x, n, w.state);
+    HPCCG_HIP_TRY(hipGetLastError());
+    HPCCG_HIP_TRY(hipEventRecord(c->ev1, c->stream));
+    e = hipStreamSynchronize(c->stream);
+    roctxRangePop();
+    HPCCG_HIP_TRY(e);
+    HPCCG_HIP_TRY(hipEventElapsedTime(&ms, c->evp, c->ev1));
+    if (t_total) *t_total = ms * 1e-3;
+    const CgState s = *c->state_h;
+    const int nit = s.k - 1;
+    if (niters) *niters = nit;
+    std::vector<double> rr(max_iter + 1);
+    HPCCG_HIP_TRY(hipMemcpy(rr.data(), w.rr_hist, (max_iter + 1) * sizeof(double), hipMemcpyDeviceToHost));
+    if (normr) *normr = std::sqrt(rr[std::max(0, nit - 1)]);
+    if (hist)
+        for (int j = 0; j <= max_iter; ++j) hist[j] = (j <= nit) ? std::sqrt(rr[j]) : -1.0;
     return s.breakdown ? HPCCG_WARN_BREAKDOWN : 0;
 }
 
@@ -805,7 +907,8 @@ int hpccg_create(hpccg_ctx **out, int device)
         hipHostMalloc(&c->scalar_h, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
         hipEventCreate(&c->ev2) != hipSuccess || hipEventCreate(&c->ev3) != hipSuccess ||
-        hipEventCreate(&c->evp) != hipSuccess) {
+        hipEventCreate(&c->evp) != hipSuccess || hipEventCreate(&c->evs0) != hipSuccess ||
+        hipEventCreate(&c->evs1) != hipSuccess) {
         set_error("hpccg_create: HIP resource allocation failed");
         hpccg_destroy(c);
         return HPCCG_ERR_HIP;
@@ -854,6 +957,8 @@ int hpccg_destroy(hpccg_ctx *c)
     if (c->ev2) hipEventDestroy(c->ev2);
     if (c->ev3) hipEventDestroy(c->ev3);
     if (c->evp) hipEventDestroy(c->evp);
+    if (c->evs0) hipEventDestroy(c->evs0);
+    if (c->evs1) hipEventDestroy(c->evs1);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return 0;
@@ -1411,6 +1516,37 @@ int hpccg_cg(hpccg_ctx *c, const double *b, double *x, int32_t max_iter, double 
     HPCCG_HIP_TRY(hipMemcpyAsync(c->s0, b, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HPCCG_HIP_TRY(hipMemcpyAsync(c->s1, x, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
     const int st = cg_device(c, c->s0, c->s1, max_iter, tolerance, niters, normr, hist, nullptr);
+    if (st < 0) return st;
+    HPCCG_TRY(hpccg_memcpy_d2h(c, x, c->s1, n * sizeof(double)));
+    return st;
+}
+
+int hpccg_cg_timed(hpccg_ctx *c, const double *b, double *x, int32_t max_iter, double tolerance, int32_t level,
+                   int32_t *niters, double *normr, double *hist, double *t_total, double *t_prepare, double *t_iter,
+                   double *t_spmv)
+{
+    HPCCG_CHECK_ARG(c && b && x, "null argument");
+    HPCCG_CHECK_ARG(level >= 1, "level must be >= 1 (the reference times nothing at TIMINGLEV 0)");
+    HPCCG_TRY(matrix_required(c));
+    HPCCG_CHECK_ARG(max_iter >= 0, "max_iter must be >= 0");
+    HPCCG_HIP_TRY(hipSetDevice(c->device));
+    const int64_t n = c->A.nrow;
+    HPCCG_TRY(ensure_stage(c, n));
+    HPCCG_TRY(ensure_workspace(c, n));
+    HPCCG_TRY(ensure_hist(c, max_iter + 2));
+    HPCCG_HIP_TRY(hipMemcpyAsync(c->s0, b, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HPCCG_HIP_TRY(hipMemcpyAsync(c->s1, x, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    int st;
+    if (level == 1) {
+        double all_ms = 0.0, loop_ms = 0.0;
+        st = cg_run(c, ctx_ws(c), true, c->s0, c->s1, max_iter, tolerance, niters, normr, hist, &all_ms, &loop_ms,
+                    true);
+        if (t_total) *t_total = loop_ms * 1e-3;
+        if (t_prepare) *t_prepare = (all_ms - loop_ms) * 1e-3;
+    } else {
+        st = cg_timed_iterations(c, c->s0, c->s1, max_iter, tolerance, niters, normr, hist, t_total, t_prepare,
+                                 t_iter, t_spmv);
+    }
     if (st < 0) return st;
     HPCCG_TRY(hpccg_memcpy_d2h(c, x, c->s1, n * sizeof(double)));
     return st;

@@ -231,6 +231,7 @@ struct hpccg_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t ev2 = nullptr, ev3 = nullptr;  // an outer solve that calls cg_device_fp64 (fall-back)
     hipEvent_t evp = nullptr;                 // end of the FP64 CG's Prepare
+    hipEvent_t evs0 = nullptr, evs1 = nullptr;  // the SpMV of one timed CG iteration
     double *scratch = nullptr;  // this device's static workspace of the no-allocation solve
 
     hpccg::MixedWs mix;

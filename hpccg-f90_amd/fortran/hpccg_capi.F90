@@ -19,7 +19,7 @@ module hpccg_capi
     public :: hpccg_context, hpccg_check, hpccg_new_tag, hpccg_resident_tag, hpccg_set_resident
     public :: c_hpccg_matrix_upload, c_hpccg_generate_matrix, c_hpccg_spmv, c_hpccg_ddot, c_hpccg_ddot1
     public :: c_hpccg_waxpby, c_hpccg_wwpby, c_hpccg_wxbw, c_hpccg_vcopy, c_hpccg_cg
-    public :: c_hpccg_compute_residual
+    public :: c_hpccg_compute_residual, c_hpccg_cg_timed
 
     interface
         integer(c_int) function c_hpccg_create(ctx, device) bind(C, name='hpccg_create')
@@ -121,6 +121,21 @@ module hpccg_capi
             integer(c_int32_t) :: niters
             real(c_double) :: normr
             real(c_double) :: hist(*)
+        end function
+        integer(c_int) function c_hpccg_cg_timed(ctx, b, x, max_iter, tolerance, level, niters, normr, hist, &
+                t_total, t_prepare, t_iter, t_spmv) bind(C, name='hpccg_cg_timed')
+            import :: c_ptr, c_int, c_int32_t, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(in) :: b(*)
+            real(c_double) :: x(*)
+            integer(c_int32_t), value :: max_iter
+            real(c_double), value :: tolerance
+            integer(c_int32_t), value :: level
+            integer(c_int32_t) :: niters
+            real(c_double) :: normr
+            real(c_double) :: hist(*)
+            real(c_double) :: t_total, t_prepare
+            real(c_double) :: t_iter(*), t_spmv(*)
         end function
     end interface
 

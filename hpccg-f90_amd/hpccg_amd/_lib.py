@@ -59,6 +59,7 @@ _SIGS = {
     "hpccg_wxbw_dev": (ctypes.c_int, [P, I32, P, D, P]),
     "hpccg_cg_dev": (ctypes.c_int, [P, P, P, I32, D, PI32, PD, P, PD]),
     "hpccg_cg_dev_noalloc": (ctypes.c_int, [P, P, P, I32, D, PI32, PD, P, PD]),
+    "hpccg_cg_timed": (ctypes.c_int, [P, P, P, I32, D, I32, PI32, PD, P, PD, PD, P, P]),
     "hpccg_cg_mixed": (ctypes.c_int, [P, P, P, I32, D, I32, PI32, PD, P, P]),
     "hpccg_cg_mixed_dev": (ctypes.c_int, [P, P, P, I32, D, I32, PI32, PD, P, P, PD]),
     "hpccg_spmv_mixed_dev": (ctypes.c_int, [P, P, P]),
@@ -328,6 +329,22 @@ class Context:
         st = _check(lib().hpccg_cg(self.h, _ptr(b), _ptr(x), max_iter, tolerance, ctypes.byref(nit),
                                    ctypes.byref(nr), _ptr(hist)), "cg")
         return x, nit.value, nr.value, hist, st
+
+    def cg_timed(self, b, x0=None, max_iter=150, tolerance=0.0, level=2):
+        """cg with the reference's timer regions -> dict(x, niters, normr, hist, status, t_total,
+        t_prepare, t_iter[niters], t_spmv[niters]) (seconds; per-iteration arrays at level >= 2)."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros(self.n) if x0 is None else np.array(x0, dtype=np.float64)
+        hist = np.zeros(max_iter + 1)
+        ti, ts = np.zeros(max(max_iter, 1)), np.zeros(max(max_iter, 1))
+        nit, nr, tt, tp = ctypes.c_int32(), ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        st = _check(lib().hpccg_cg_timed(self.h, _ptr(b), _ptr(x), max_iter, tolerance, level, ctypes.byref(nit),
+                                         ctypes.byref(nr), _ptr(hist), ctypes.byref(tt), ctypes.byref(tp),
+                                         _ptr(ti), _ptr(ts)), "cg_timed")
+        k = nit.value
+        return {"x": x, "niters": k, "normr": nr.value, "hist": hist, "status": st, "t_total": tt.value,
+                "t_prepare": tp.value, "t_iter": ti[:k] if level >= 2 else None,
+                "t_spmv": ts[:k] if level >= 2 else None}
 
     # ---- device-array entry points
     def cg_dev(self, b: DeviceArray, x: DeviceArray, max_iter=150, tolerance=0.0, want_hist=True):

@@ -115,6 +115,20 @@ int hpccg_compute_residual(hpccg_ctx *ctx, int32_t n, const double *v1, const do
 int hpccg_cg(hpccg_ctx *ctx, const double *b, double *x, int32_t max_iter, double tolerance,
              int32_t *niters, double *normr, double *hist);
 
+/* cg (host arrays, as hpccg_cg) with the reference's timer regions measured on the device
+ * (numerics.F90:91-156, TIMER_* levels of MPI/includes/timer.h:40-58).  Times in seconds.
+ *   level 1:  the graph-replayed solve; *t_total = the iteration loop with its final x update
+ *             (the "Total" region, numerics.F90:106,149), *t_prepare = Prepare (:91-100).
+ *   level >= 2: iterations launched one at a time with the stopping test on the host before
+ *             each (numerics.F90:109): t_iter[k-1] = iteration k ("Iteration", :113,142),
+ *             t_spmv[k-1] = its SpMV ("spmv", :132-134), for k = 1..niters (arrays of max_iter,
+ *             may be NULL); *t_total, *t_prepare as above.
+ * roctx ranges named "Prepare", "Total", "Iteration" (level >= 2) and "spmv" (level >= 2)
+ * bracket the same device work (rocprofv3 --marker-trace).  Results are hpccg_cg's bit for bit. */
+int hpccg_cg_timed(hpccg_ctx *ctx, const double *b, double *x, int32_t max_iter, double tolerance,
+                   int32_t level, int32_t *niters, double *normr, double *hist, double *t_total,
+                   double *t_prepare, double *t_iter, double *t_spmv);
+
 /* ---- kernels, device arrays --------------------------------------------------------------- */
 int hpccg_spmv_dev(hpccg_ctx *ctx, const double *x, double *y);
 int hpccg_ddot_dev(hpccg_ctx *ctx, int32_t n, const double *x, const double *y, double *r);

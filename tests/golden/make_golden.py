@@ -8,6 +8,7 @@ only (inputs described by grid and seed, outputs as exact float reprs and SHA-25
 
     python tests/golden/make_golden.py [--big]     # --big adds 128^3 / 256^3 cases (minutes)
     python tests/golden/make_golden.py --exact     # only the exact-dot 128^3 history (~15 s)
+    python tests/golden/make_golden.py --timers    # only the TIMINGLEV=3 stdout (timer values masked)
 
 The exact-dot history (cg_exacthist_*.json) is written by the C restatement (oracle.cg with
 exact_dots=True: the reference's cg loop with both dots rounded once from a double-double
@@ -184,6 +185,26 @@ def exact_hist_case(nx, ny, nz, kmax):
             "xerr_max": float(np.max(np.abs(x - 1.0)))}
 
 
+def mask_timers(lines):
+    """Stdout lines with the measured numbers replaced: TIMER values by <t>, MFLOPS by <r>."""
+    out = []
+    for ln in lines:
+        if ln.strip().startswith("TIMER:"):
+            ln = re.sub(r"(TIMER:\s+\S+\s+)\S+(\s+sec)", r"\1<t>\2", ln)
+        elif ln.strip().startswith("MFLOPS SPARSEMV"):
+            ln = re.sub(r"(MFLOPS SPARSEMV\s+)\S+", r"\1<r>", ln)
+        out.append(ln)
+    return out
+
+
+def write_timers():
+    """stdout of the reference built with -DTIMINGLEV=3 (every timer region on) at 16^3."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "serial3", "HPCCG")
+    out = subprocess.run([exe, "16", "16", "16"], check=True, capture_output=True, text=True).stdout
+    with open(os.path.join(GOLD, "ref_stdout_t3_16x16x16.txt"), "w") as f:
+        f.write("\n".join(mask_timers(out.splitlines())) + "\n")
+
+
 def write_exact():
     rec = exact_hist_case(128, 128, 128, 150)
     with open(os.path.join(GOLD, "cg_exacthist_128x128x128_ones.json"), "w") as f:
@@ -194,10 +215,15 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--big", action="store_true")
     ap.add_argument("--exact", action="store_true", help="only write the exact-dot 128^3 history")
+    ap.add_argument("--timers", action="store_true", help="only write the TIMINGLEV=3 stdout fixture")
     args = ap.parse_args()
     if args.exact:
         write_exact()
         print("exact-dot history written to", GOLD)
+        return
+    if args.timers:
+        write_timers()
+        print("TIMINGLEV=3 stdout written to", GOLD)
         return
     if not os.path.exists(HARNESS):
         sys.exit("oracle/_ref not built: run `make -C oracle ref` (needs the reference sources)")
@@ -229,6 +255,7 @@ def main():
         json.dump({"source": "reference cg via oracle/_ref/serial/ref_harness solve, b = reference spmv(x*)",
                    "cases": solves}, f, indent=1)
 
+    write_timers()
     out = subprocess.run([MAIN, "32", "32", "32"], check=True, capture_output=True, text=True).stdout
     lines = [ln for ln in out.splitlines() if "TIMER" not in ln]
     with open(os.path.join(GOLD, "ref_stdout_32x32x32.txt"), "w") as f:
