@@ -27,8 +27,8 @@ struct TrainData {
 struct CnnNet {
     float *W = nullptr;              // FP32 master weights: W1 | W2 | W3
     float *m = nullptr, *v = nullptr;  // Adam moments
-    bf16x8 *pk_f1 = nullptr, *pk_f2 = nullptr, *pk_f3 = nullptr;  // forward operands
-    bf16x8 *pk_f3f = nullptr;                                     // layer 3, fused-kernel form
+    bf16x8 *pk_f1 = nullptr, *pk_f2 = nullptr;                    // forward operands
+    bf16x8 *pk_f3f = nullptr;                                     // layer 3, banded form (both forward kernels)
     bf16x8 *pk_b3 = nullptr, *pk_b2 = nullptr;                    // backward-data operands
     int *step = nullptr;             // device training step counter
     TrainData data;
@@ -53,13 +53,13 @@ static CnnNet *net_of(hpccg_ctx *c) { return (CnnNet *)c->ext[1]; }
 // ---- weight packing into MFMA A-operands -------------------------------------------------------
 // pack_f1[l][j] = W1[co=l%16][t=k1_tap(l/16, j)];
 // pack_f2[s][l][j] = W2[co=l%16][ci=8((l/16)&1)+j][t=zk_tap(s, (l/16)/2)] (z-major K order);
-// pack_f3[s][l][j] = W3[ci][t] in row co = s/5 only (layer 3, layer-wise: row 1+dz takes the
-//   steps of plane offset dz), pack_f3f[p][l][j] = W3[ci][t = 9 co + 2p + (l/16)/2] for rows
-//   co = 0..2 (the fused kernel's one-plane form: row co is plane offset co-1);
+// pack_f3f[s][l][j] = W3[ci][t] of the banded layer 3 (cnn_kernels.hpp, NSTEP3B): MFMA row
+//   m = l%16 = 4 dzi + jr, K slot (window row s/3 + 2 (l/16 >= 2), x-offset s%3, ci =
+//   8((l/16)&1)+j), t = 9 dzi + 3 (row - jr) + x-offset where 0 <= row - jr <= 2, else 0;
 // pack_b3[l][j] = W3[l%16][26 - k1_tap(l/16, j)];
 // pack_b2[s][l][j] = W2[ci'][co'=l%16][26 - t] with ci' = 8((l/16)&1)+j (transposed, flipped).
-__global__ void k_pack(const float *__restrict__ W, uint16_t *f1, uint16_t *f2, uint16_t *f3, uint16_t *f3f,
-                       uint16_t *b3, uint16_t *b2)
+__global__ void k_pack(const float *__restrict__ W, uint16_t *f1, uint16_t *f2, uint16_t *f3f, uint16_t *b3,
+                       uint16_t *b2)
 {
     const float *W1 = W, *W2 = W + NW1, *W3 = W + NW1 + NW2;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -72,13 +72,15 @@ __global__ void k_pack(const float *__restrict__ W, uint16_t *f1, uint16_t *f2, 
         const int s = i / 512, l = (i / 8) % 64, j = i % 8;
         const int co = l % 16, ci = 8 * ((l / 16) & 1) + j, t = zk_tap(s, (l / 16) / 2);
         f2[i] = t >= 0 ? f2bf(W2[(co * 16 + ci) * 27 + t]) : 0;
-        f3[i] = (t >= 0 && co == s / 5) ? f2bf(W3[ci * 27 + t]) : 0;
         b2[i] = t >= 0 ? f2bf(W2[(ci * 16 + co) * 27 + (26 - t)]) : 0;
     }
-    if (i < NSTEP3F * 64 * 8) {
-        const int p = i / 512, l = (i / 8) % 64, j = i % 8;
-        const int co = l % 16, ci = 8 * ((l / 16) & 1) + j, t9 = 2 * p + (l / 16) / 2;
-        f3f[i] = (co < 3 && t9 < 9) ? f2bf(W3[ci * 27 + 9 * co + t9]) : 0;
+    if (i < NSTEP3B * 64 * 8) {  // banded layer 3
+        const int s = i / 512, l = (i / 8) % 64, j = i % 8;
+        const int m = l % 16, cl = l / 16, ci = 8 * (cl & 1) + j;
+        const int row = s / 3 + (cl >= 2 ? L3ROWS : 0), dxi = s % 3;  // window row and x-offset + 1 of this K slot
+        const int dzi = m / 4, jr = m % 4, dyi = row - jr;  // MFMA row m = 4 dzi + jr
+        const bool on = dzi < 3 && jr < L3ROWS && dyi >= 0 && dyi <= 2;
+        f3f[i] = on ? f2bf(W3[ci * 27 + 9 * dzi + 3 * dyi + dxi]) : 0;
     }
 }
 
@@ -125,7 +127,7 @@ static int ensure_net(hpccg_ctx *c)
         dmalloc(&N->v, NWALL * 4) != hipSuccess || dmalloc(&N->grad, NWALL * 4) != hipSuccess ||
         dmalloc(&N->pk_f1, 64 * 16) != hipSuccess || dmalloc(&N->pk_b3, 64 * 16) != hipSuccess ||
         dmalloc(&N->pk_f2, NSTEP16 * 64 * 16) != hipSuccess ||
-        dmalloc(&N->pk_f3, NSTEP16 * 64 * 16) != hipSuccess || dmalloc(&N->pk_f3f, NSTEP3F * 64 * 16) != hipSuccess ||
+        dmalloc(&N->pk_f3f, NSTEP3B * 64 * 16) != hipSuccess ||
         dmalloc(&N->pk_b2, NSTEP16 * 64 * 16) != hipSuccess || dmalloc(&N->step, sizeof(int)) != hipSuccess ||
         dmalloc(&N->scale_in, 4096 * 4) != hipSuccess || dmalloc(&N->scale_out, 4096 * 4) != hipSuccess) {
         set_error("cnn: allocation failed");
@@ -144,7 +146,7 @@ static int repack(hpccg_ctx *c)
 {
     CnnNet *N = net_of(c);
     hipLaunchKernelGGL(k_pack, dim3((NSTEP16 * 512 + 255) / 256), dim3(256), 0, c->stream, N->W, (uint16_t *)N->pk_f1,
-                       (uint16_t *)N->pk_f2, (uint16_t *)N->pk_f3, (uint16_t *)N->pk_f3f, (uint16_t *)N->pk_b3,
+                       (uint16_t *)N->pk_f2, (uint16_t *)N->pk_f3f, (uint16_t *)N->pk_b3,
                        (uint16_t *)N->pk_b2);
     HPCCG_HIP_TRY(hipGetLastError());
     return 0;
@@ -195,7 +197,7 @@ static void forward(hpccg_ctx *c, const float *in, const float *s_in, float *z, 
     CnnNet *N = net_of(c);
     conv<IN_F32_1, OUT_RELU_BF16>(c, in, s_in, N->pk_f1, N->h1, nullptr, nullptr, g, batch, skip);
     conv<IN_BF16_16, OUT_RELU_BF16>(c, N->h1, nullptr, N->pk_f2, N->h2, nullptr, nullptr, g, batch, skip);
-    conv<IN_BF16_16, OUT_F32_DZ3>(c, N->h2, nullptr, N->pk_f3, z, nullptr, s_out, g, batch, skip);
+    hipLaunchKernelGGL(k_conv3, dim3(g.ntiles() * batch), dim3(256), 0, c->stream, N->h2, N->pk_f3f, z, s_out, g, skip);
 }
 
 // inference on one field: the fused kernel (bitwise equal to forward(), no activation traffic)
@@ -204,7 +206,7 @@ static void forward_fused(hpccg_ctx *c, const float *in, const float *s_in, floa
 {
     CnnNet *N = net_of(c);
     const int tx = (g.nx + FX - 1) / FX, ty = (g.ny + FY - 1) / FY;
-    // about one workgroup per CU (registers admit one 12-wave workgroup per CU), z-chunks of at
+    // about one workgroup per CU (registers admit one 8-wave workgroup per CU), z-chunks of at
     // least 8 planes (each chunk re-reads 6 planes of halo)
     const int want = std::max(1, (c->info.num_cu + tx * ty - 1) / (tx * ty));
     const int nzc = std::max(1, std::min(want, (g.nz + 7) / 8));
@@ -683,7 +685,7 @@ void free_learned(hpccg_ctx *c)
     if (CnnNet *N = net_of(c)) {
         if (N->train_graph) hipGraphExecDestroy(N->train_graph);
         for (void *p : {(void *)N->W, (void *)N->m, (void *)N->v, (void *)N->pk_f1, (void *)N->pk_f2,
-                        (void *)N->pk_f3, (void *)N->pk_f3f, (void *)N->pk_b3, (void *)N->pk_b2, (void *)N->step, (void *)N->h1,
+                        (void *)N->pk_f3f, (void *)N->pk_b3, (void *)N->pk_b2, (void *)N->step, (void *)N->h1,
                         (void *)N->h2, (void *)N->dh2, (void *)N->dh1, (void *)N->dz, (void *)N->in, (void *)N->z,
                         (void *)N->tgt, (void *)N->wbuf, (void *)N->scale_in, (void *)N->scale_out,
                         (void *)N->partials, (void *)N->grad, (void *)N->loss, (void *)N->lpart,
@@ -987,7 +989,7 @@ int hpccg_cnn_train(hpccg_ctx *c, int32_t steps, int32_t batch, float lr, uint64
     hipLaunchKernelGGL(k_adam, dim3((NWALL + 255) / 256), dim3(256), 0, c->stream, N->W, N->m, N->v, N->grad, NWALL,
                        lr, steps, N->step);
     hipLaunchKernelGGL(k_pack, dim3((NSTEP16 * 512 + 255) / 256), dim3(256), 0, c->stream, N->W,
-                       (uint16_t *)N->pk_f1, (uint16_t *)N->pk_f2, (uint16_t *)N->pk_f3, (uint16_t *)N->pk_f3f,
+                       (uint16_t *)N->pk_f1, (uint16_t *)N->pk_f2, (uint16_t *)N->pk_f3f,
                        (uint16_t *)N->pk_b3, (uint16_t *)N->pk_b2);
     hipLaunchKernelGGL(k_step_inc, dim3(1), dim3(64), 0, c->stream, N->step);
     const hipError_t ec = hipStreamEndCapture(c->stream, &gr);

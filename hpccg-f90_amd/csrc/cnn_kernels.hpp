@@ -44,7 +44,17 @@ constexpr int HX = TX + 2, HY = TY + 2, HZ = TZ + 2;
 constexpr int HV = HX * HY * HZ;  // 600 halo voxels
 constexpr int TV = TX * TY * TZ;  // 256 voxels per tile
 constexpr int NSTEP16 = 15;       // MFMA K-steps for 16 input channels x 27 taps (z-major)
-constexpr int NSTEP3F = 5;        // K-steps of one input plane for the three-row layer-3 form
+constexpr int NSTEP3F = 5;        // in-plane tap pairs of one input plane (z-major K order)
+// Banded form of the one-channel layer 3 (fused inference and the layer-wise forward alike): one
+// MFMA block is L3ROWS = 2 output rows x 16 voxels (the N columns) of one h2 plane, with K = a
+// window of 4 h2 rows x 3 x-offsets x 16 channels = 192 = 6 K-steps.  K-step s pairs the window
+// (row, x-offset) (s / 3, s % 3) for operand rows k = 0..15 with (s / 3 + 2, s % 3) for k = 16..31.
+// MFMA row m = 4 dzi + j (dzi < 3, j < 2) holds the plane-offset dz = dzi - 1 partial of
+// output row j, whose taps are window rows j .. j+2 (dy = row - j - 1); the other rows are zero.
+// So a lane holds in register j the dz = c - 1 partial of row j (c = lane / 16), and an output
+// voxel is ((P(dz=-1) + P(0)) + P(+1)) of three blocks on consecutive h2 planes.
+constexpr int NSTEP3B = 6;
+constexpr int L3ROWS = 2;
 constexpr int NW1 = 16 * 27, NW2 = 16 * 16 * 27, NW3 = 16 * 27;
 constexpr int NWALL = NW1 + NW2 + NW3;  // 7776 parameters
 
@@ -109,12 +119,20 @@ __device__ inline f32x4 mfma16(bf16x8 a, bf16x8 b, f32x4 c)
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-// ReLU then BF16 of the four accumulator rows of a lane, packed as 4 x 16 bits
+// ReLU of two BF16 values in one word: a BF16 number is negative exactly when its bit pattern
+// is negative as an int16, so one packed integer max with 0 zeroes the negatives (-0 included).
+// Rounding to BF16 preserves signs, so bf16(max(a, 0)) == max(bf16(a), 0) for every non-NaN a.
+typedef short i16x2 __attribute__((ext_vector_type(2)));
+__device__ inline uint32_t relu_pk(uint32_t p)
+{
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(i16x2, p), i16x2{0, 0}));
+}
+// BF16 then ReLU of the four accumulator rows of a lane, packed as 4 x 16 bits
 __device__ inline u32x2 relu_bf16x4(f32x4 a)
 {
     u32x2 pk;
-    pk[0] = pack_bf16(__builtin_fmaxf(a[0], 0.f), __builtin_fmaxf(a[1], 0.f));
-    pk[1] = pack_bf16(__builtin_fmaxf(a[2], 0.f), __builtin_fmaxf(a[3], 0.f));
+    pk[0] = relu_pk(pack_bf16(a[0], a[1]));
+    pk[1] = relu_pk(pack_bf16(a[2], a[3]));
     return pk;
 }
 
@@ -133,10 +151,9 @@ inline Grid3 make_grid(int nx, int ny, int nz)
 // Input of a convolution: BF16 channel-last activations, or (layer 1) an FP32 field multiplied
 // by a per-sample scale and rounded to BF16 while it is staged.
 enum InKind { IN_BF16_16 = 0, IN_BF16_1 = 1, IN_F32_1 = 2 };
-// Epilogue: ReLU -> BF16 [v][16]; FP32 ((row0 + row1) + row2) times a per-sample scale (the
-// one-channel layer 3 from its three plane-offset partials); masked BF16 (the backward pass
-// through a ReLU whose forward output is `mask`).
-enum OutKind { OUT_RELU_BF16 = 0, OUT_F32_DZ3 = 1, OUT_MASKED_BF16 = 2 };
+// Epilogue: ReLU -> BF16 [v][16]; masked BF16 (the backward pass through a ReLU whose forward
+// output is `mask`).  The one-channel layer 3 of the forward pass has its own kernel (k_conv3).
+enum OutKind { OUT_RELU_BF16 = 0, OUT_MASKED_BF16 = 2 };
 
 // Stage the 10x10x6 halo of tile (ox,oy,oz) of sample b into LDS (channel-last, 16 or 1 ch).
 template <int IK>
@@ -171,8 +188,8 @@ __device__ inline void stage_halo(const void *in, float scale, uint16_t *sm, con
     }
 }
 
-// One 3x3x3 convolution layer with 16 output channels (layer 3: its single output channel as
-// three plane-offset partials in rows 0..2).  Grid: one workgroup per (tile, sample).
+// One 3x3x3 convolution layer with 16 output channels, or the backward pass through one.
+// Grid: one workgroup per (tile, sample).
 template <int IK, int OK>
 __global__ __launch_bounds__(256) void k_conv(const void *__restrict__ in, const float *__restrict__ in_scale,
                                               const bf16x8 *__restrict__ wpack, void *__restrict__ out,
@@ -235,9 +252,7 @@ __global__ __launch_bounds__(256) void k_conv(const void *__restrict__ in, const
         const int gx = ox + x, gy = oy + y, gz = oz + z;
         if (gx < g.nx && gy < g.ny && gz < g.nz) {
             const int64_t gv = (int64_t)b * N + ((int64_t)gz * g.ny + gy) * g.nx + gx;
-            if (OK == OUT_F32_DZ3) {
-                if (c == 0) ((float *)out)[gv] = ((acc[0] + acc[1]) + acc[2]) * (out_scale ? out_scale[b] : 1.0f);
-            } else {
+            {
                 u32x2 pk;
                 if (OK == OUT_RELU_BF16) {
                     pk = relu_bf16x4(acc);
@@ -253,6 +268,61 @@ __global__ __launch_bounds__(256) void k_conv(const void *__restrict__ in, const
                 }
                 *(u32x2 *)((uint16_t *)out + gv * 16 + 4 * c) = pk;
             }
+        }
+    }
+}
+
+// The one-channel layer 3 of the forward pass, z = conv(h2; W3) times a per-sample scale, in the
+// banded form (NSTEP3B): the MFMA columns of a tile are (x, row block, h2 plane) -- 8 x 4 x 6 =
+// 192, twelve groups of 16, three per wave -- each six MFMAs on the staged h2 halo; the
+// partials go to LDS and every output voxel adds its three in the fused kernel's order,
+// ((P(-1) + P(0)) + P(+1)), so this is bitwise the fused inference.
+__global__ __launch_bounds__(256) void k_conv3(const uint16_t *__restrict__ in, const bf16x8 *__restrict__ wpack,
+                                               float *__restrict__ out, const float *__restrict__ out_scale, Grid3 g,
+                                               const int *__restrict__ skip)
+{
+    if (skip && *skip) return;
+    constexpr int NB = TY / L3ROWS;  // row blocks per tile
+    __shared__ __attribute__((aligned(16))) uint16_t sm[HV * 16];
+    __shared__ float P[HZ][NB][L3ROWS][3][TX];  // partial of (h2 plane, block, row, dz + 1, x)
+    const int nt = g.ntiles();
+    const int b = blockIdx.x / nt, tile = blockIdx.x % nt;
+    const int ox = (tile % g.tx) * TX, oy = ((tile / g.tx) % g.ty) * TY, oz = (tile / (g.tx * g.ty)) * TZ;
+    stage_halo<IN_BF16_16>(in, 1.0f, sm, g, b, ox, oy, oz);
+    const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+    const int c = lane / 16, n = lane % 16;
+    const bool hi = c >= 2;
+    const int half = (c & 1) * 8;
+    bf16x8 A[NSTEP3B];
+#pragma unroll
+    for (int s = 0; s < NSTEP3B; ++s) A[s] = wpack[s * 64 + lane];
+    __syncthreads();
+    constexpr int GROUPS = TX * NB * HZ / 16;
+#pragma unroll 1
+    for (int gi = w; gi < GROUPS; gi += 4) {
+        const int id = 16 * gi + n;
+        const int x = id % TX, bk = (id / TX) % NB, hz = id / (TX * NB);
+        const int base = ((hz * HY + L3ROWS * bk + (hi ? L3ROWS : 0)) * HX + x) * 16 + half;
+        f32x4 d = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < NSTEP3B; ++s)
+            d = mfma16(A[s], __builtin_bit_cast(bf16x8, *(const u32x4 *)(sm + base + ((s / 3) * HX + s % 3) * 16)), d);
+        if (c < 3) {
+#pragma unroll
+            for (int j = 0; j < L3ROWS; ++j) P[hz][bk][j][c][x] = d[j];
+        }
+    }
+    __syncthreads();
+    const int64_t N = g.nvox();
+    const float scale = out_scale ? out_scale[b] : 1.0f;
+    {
+        const int v = threadIdx.x;  // one output voxel of the 8 x 8 x 4 tile per thread
+        const int x = v % TX, y = (v / TX) % TY, z = v / (TX * TY);
+        const int bk = y / L3ROWS, j = y % L3ROWS;
+        const int gx = ox + x, gy = oy + y, gz = oz + z;
+        if (gx < g.nx && gy < g.ny && gz < g.nz) {
+            const float r = (P[z][bk][j][0][x] + P[z + 1][bk][j][1][x]) + P[z + 2][bk][j][2][x];
+            out[(int64_t)b * N + ((int64_t)gz * g.ny + gy) * g.nx + gx] = r * scale;
         }
     }
 }
@@ -340,35 +410,41 @@ __global__ __launch_bounds__(256) void k_wgrad(const uint16_t *__restrict__ dout
 }
 
 // ---- fused inference: the three layers in one kernel -------------------------------------------
-// A workgroup owns a 16 x 16 (x, y) column of output voxels and a range of z-planes and marches
-// along z, one plane per step.  In step i every wave does its share of
+// A workgroup of 8 waves (2 per SIMD, up to 256 VGPRs each) owns a 16 x 16 (x, y) column of
+// output voxels and a range of z-planes and marches along z, one plane per step.  In step i every
+// wave does its share of
 //   layer 1: h1 plane i-1 from the input window (planes i-2 .. i; one MFMA per 16 voxels,
-//            K = 27 taps);
+//            K = 27 taps; 25 groups, 3 per wave and a 25th on wave 0);
 //   layer 2: reads h1 plane i-2 ONCE -- five operands per 16 voxels, one per in-plane tap
 //            pair -- and adds it to the three h2 planes it touches (15 MFMAs), h2 plane i-3
-//            being complete afterwards;
-//   layer 3: reads h2 plane i-4 once the same way (five MFMAs, the three plane-offset partials
-//            in rows 0..2) and completes output plane i-5;
+//            being complete afterwards (21 groups in 24 slots, 3 per wave);
+//   layer 3: one banded block per wave (2 output rows, see NSTEP3B): six MFMAs on h2 plane i-4,
+//            whose three plane-offset partials move one 16-lane group per step (a shuffle), so
+//            output plane i-5 is complete;
 //   input:   stores plane i+1 (rounded to BF16, loaded three steps earlier) into a four-plane
 //            ring and requests plane i+4.
 // The layers of one step touch different planes -- h1 and h2 are double-buffered in LDS -- so a
-// step needs a single barrier and a wave's three layers overlap.  The h2 and output partial
-// sums of unfinished planes stay in registers, rotating through three slots (the loop body is
-// unrolled three times so that the slots are compile-time registers).  h1 is computed on a
+// step needs a single barrier.  A step of the march's steady state (every layer active: all
+// steps but the first seven and the last two) is one basic block, so the compiler can overlap a
+// wave's three layers: it has no plane-range tests, the spare layer-2 slots compute a clamped
+// group, and lanes with nothing to store write to a scratch slot instead of branching.  The h2
+// partial sums of unfinished planes stay in registers, rotating through three slots (the march
+// is unrolled three times so that the slots are compile-time registers).  h1 is computed on a
 // 20 x 20 and h2 on an 18 x 18 footprint (the halos the next layer needs).  Every LDS address
 // and grid-boundary test that does not depend on the plane is formed once, before the march.
-// Each accumulator receives its K-steps in the order of the layer-wise k_conv (z-major), so the
-// output is bitwise the three-kernel forward pass's.
+// Each accumulator receives its K-steps in the order of the layer-wise k_conv, so the output is
+// bitwise the three-kernel forward pass's.
 constexpr int FX = 16, FY = 16;             // output footprint per workgroup
 constexpr int F2X = FX + 2, F2Y = FY + 2;   // h2 footprint (324 voxels, 21 groups of 16)
 constexpr int F1X = FX + 4, F1Y = FY + 4;   // h1 footprint (400 voxels, 25 groups)
 constexpr int F0X = FX + 6, F0Y = FY + 6;   // input footprint (484 voxels)
 constexpr int F2V = F2X * F2Y, F1V = F1X * F1Y, F0V = F0X * F0Y;
 constexpr int F2G = (F2V + 15) / 16, F1G = F1V / 16, F3G = FX * FY / 16;
-constexpr int FWAVES = 12, FTHREADS = 64 * FWAVES;  // waves per fused workgroup (3 per SIMD)
-constexpr int G1W = (F1G + FWAVES - 1) / FWAVES;    // voxel groups per wave: layer 1 (at most 3),
-constexpr int G2W = (F2G + FWAVES - 1) / FWAVES;    //   layer 2 (2),
-constexpr int G3W = (F3G + FWAVES - 1) / FWAVES;    //   layer 3 (2)
+constexpr int FWAVES = 8, FTHREADS = 64 * FWAVES;  // waves per fused workgroup (2 per SIMD)
+constexpr int G1W = (F1G + FWAVES - 1) / FWAVES;    // voxel groups per wave: layer 1 (at most 4),
+constexpr int G2W = (F2G + FWAVES - 1) / FWAVES;    //   layer 2 (3),
+constexpr int F3B = FY / L3ROWS;                   //   layer 3: row blocks (one per wave)
+static_assert(F3B == FWAVES, "one layer-3 row block per wave");
 constexpr int TPL = F0Y * F1X;                     // x-triplets per input-window plane (440)
 constexpr int VS = 16;  // h1 / h2 voxel record in LDS: 16 BF16 channels (32 bytes)
 constexpr int H1S = F1V * VS + 8, H2S = F2V * VS + 8;  // plane buffers + 8 zero channels
@@ -376,7 +452,7 @@ constexpr int H1S = F1V * VS + 8, H2S = F2V * VS + 8;  // plane buffers + 8 zero
 template <int K>
 using Slot = std::integral_constant<int, K>;
 
-__global__ __launch_bounds__(FTHREADS) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_fused_fwd(
+__global__ __launch_bounds__(FTHREADS) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_fused_fwd(
     const float *__restrict__ in, const float *__restrict__ in_scale, const bf16x8 *__restrict__ pk1,
     const bf16x8 *__restrict__ pk2, const bf16x8 *__restrict__ pk3f, float *__restrict__ out,
     const float *__restrict__ out_scale, int nx, int ny, int nz, int tiles_x, int tiles_y, int zchunk,
@@ -390,6 +466,8 @@ __global__ __launch_bounds__(FTHREADS) __attribute__((amdgpu_waves_per_eu(3, 3))
     // empty tenth tap slot)
     __shared__ __attribute__((aligned(16))) uint16_t H1[2][H1S];
     __shared__ __attribute__((aligned(16))) uint16_t H2[2][H2S];
+    // lanes with nothing to store write here, so that the stores need no branch
+    __shared__ __attribute__((aligned(16))) u32x2 Trash[FTHREADS];
     const int txy = blockIdx.x % (tiles_x * tiles_y), zc = blockIdx.x / (tiles_x * tiles_y);
     const int ox = (txy % tiles_x) * FX, oy = (txy / tiles_x) * FY;
     const int z0 = zc * zchunk, z1 = min(z0 + zchunk, nz);
@@ -406,8 +484,8 @@ __global__ __launch_bounds__(FTHREADS) __attribute__((amdgpu_waves_per_eu(3, 3))
     bf16x8 A2[NSTEP16];
 #pragma unroll
     for (int s = 0; s < NSTEP16; ++s) A2[s] = pk2[s * 64 + lane];
-    __shared__ bf16x8 A3s[NSTEP3F * 64];
-    for (int t = threadIdx.x; t < NSTEP3F * 64; t += FTHREADS) A3s[t] = pk3f[t];
+    __shared__ bf16x8 A3s[NSTEP3B * 64];
+    for (int t = threadIdx.x; t < NSTEP3B * 64; t += FTHREADS) A3s[t] = pk3f[t];
     if (threadIdx.x < 8) {
         H1[0][F1V * VS + threadIdx.x] = 0;
         H1[1][F1V * VS + threadIdx.x] = 0;
@@ -439,13 +517,15 @@ __global__ __launch_bounds__(FTHREADS) __attribute__((amdgpu_waves_per_eu(3, 3))
         const int gx = ox - 1 + hx, gy = oy - 1 + hy;
         if (g < F2G && v < F2V && gx >= 0 && gx < nx && gy >= 0 && gy < ny) ok2 |= 1u << k;
     }
-    // layer 3: group k is output row y = w + 8k
-    const int b3 = ((w * F2X) + n) * VS + half + hiv;
-    const int64_t ob = (int64_t)(oy + w) * nx + ox + n;
-    unsigned ok3 = 0;
+    // layer 3: wave w owns the row block of output rows 2w, 2w+1 (h2 window rows 2w .. 2w+3,
+    // operand rows 16..31 two rows lower)
+    const int bk = w;
+    const int b3 = ((L3ROWS * bk + (hi ? L3ROWS : 0)) * F2X + n) * VS + half;
+    const int64_t ob = (int64_t)(oy + L3ROWS * bk) * nx + ox + n;
+    unsigned ok3 = 0;  // bit j: output row 2 bk + j of this lane's voxel lies inside the grid
 #pragma unroll
-    for (int k = 0; k < G3W; ++k)
-        if (w + FWAVES * k < F3G && c == 0 && ox + n < nx && oy + w + FWAVES * k < ny) ok3 |= 1u << k;
+    for (int j = 0; j < L3ROWS; ++j)
+        if (ox + n < nx && oy + L3ROWS * bk + j < ny) ok3 |= 1u << j;
 
     // input loader: thread t < TPL owns triplet t (window row t / F1X, columns t % F1X .. +2).
     // The three loads read clamped, always valid addresses without branches, so they stay in
@@ -487,37 +567,38 @@ __global__ __launch_bounds__(FTHREADS) __attribute__((amdgpu_waves_per_eu(3, 3))
     };
 
     f32x4 acc2[3][G2W];
-    float acc3[3][G3W];
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
 #pragma unroll
         for (int k = 0; k < G2W; ++k) acc2[r][k] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int k = 0; k < G3W; ++k) acc3[r][k] = 0.f;
     }
+    // layer-3 running sums: lanes 16 dzi + n hold, for the 2 rows of the block, the sum over
+    // plane offsets -1 .. dzi - 1 of the output plane those offsets have reached
+    f32x4 run3 = {0.f, 0.f, 0.f, 0.f};
     // input planes are requested three steps before they are stored (step i stores plane i+1,
     // loaded during step i-3), so the global latency hides behind three steps of work
     F3 pre0 = fetch(z0 - 3), pre1 = fetch(z0 - 2), pre2 = fetch(z0 - 1);
 
     // step i; rotating slot R holds the partial sums the step starts (h2 plane i-1, output plane
     // i-3), R+2 those it continues, R+1 those it completes
-    auto step = [&](auto R, int i) {
+    auto step = [&](auto R, int i, auto steadyc) __attribute__((always_inline)) {
         constexpr int NXT = decltype(R)::value, CUR = (NXT + 2) % 3, PRV = (NXT + 1) % 3;
-        if (trip) T[(i + 1) & 3][threadIdx.x] = to_triplet(i + 1, pre0);
+        constexpr bool STEADY = decltype(steadyc)::value;  // every layer active: no range tests
+        *(trip ? &T[(i + 1) & 3][threadIdx.x] : &Trash[threadIdx.x]) = to_triplet(i + 1, pre0);
         pre0 = pre1;
         pre1 = pre2;
         pre2 = fetch(i + 4);
         // layer 2 first (most MFMA work; layers 1 and 3 fill its gaps):
         // h1 plane q2 = i-2 feeds h2 planes q2+1 (NXT), q2 (CUR) and q2-1 (PRV, done)
         const int q2 = i - 2;
-        if (q2 >= z0 - 2 && q2 <= z1 + 1) {
+        if (STEADY || (q2 >= z0 - 2 && q2 <= z1 + 1)) {
             const int pd = q2 - 1;
             const bool keep = pd >= z0 - 1, zin = pd >= 0 && pd < nz;
             const uint16_t *Hq = H1[q2 & 1];
             uint16_t *Hw = H2[pd & 1] + v1 * VS + 4 * c;
 #pragma unroll
             for (int k = 0; k < G2W; ++k) {
-                if (w + FWAVES * k < F2G) {
+                {  // groups past the footprint compute a clamped group and store nothing
                     u32x4 raw[NSTEP3F];
                     load_pairs(Hq, std::integral_constant<int, F1X>{}, F1V * VS, b2[k], raw);
                     f32x4 an = {0.f, 0.f, 0.f, 0.f};
@@ -530,20 +611,23 @@ __global__ __launch_bounds__(FTHREADS) __attribute__((amdgpu_waves_per_eu(3, 3))
                     }
                     u32x2 pk = relu_bf16x4(acc2[PRV][k]);
                     acc2[NXT][k] = an;
-                    if (!(zin && ((ok2 >> k) & 1u))) pk[0] = pk[1] = 0u;
-                    if (keep && 16 * (w + FWAVES * k) + n < F2V) *(u32x2 *)(Hw + 16 * FWAVES * k * VS) = pk;
+                    const uint32_t m2 = (zin && ((ok2 >> k) & 1u)) ? 0xffffffffu : 0u;
+                    pk[0] &= m2;
+                    pk[1] &= m2;
+                    const bool st2 = keep && 16 * (w + FWAVES * k) + n < F2V;
+                    *(st2 ? (u32x2 *)(Hw + 16 * FWAVES * k * VS) : &Trash[threadIdx.x]) = pk;
                 }
             }
         }
         // layer 1: h1 plane q1 = i-1 from input planes q1-1, q1, q1+1
         const int q1 = i - 1;
-        if (q1 >= z0 - 2 && q1 <= z1 + 1) {
+        if (STEADY || (q1 >= z0 - 2 && q1 <= z1 + 1)) {
             const bool zin = q1 >= 0 && q1 < nz;
             const u32x2 *TA = T[(q1 - 1 + pA) & 3] + rA, *TB = T[(q1 - 1 + pB) & 3] + rB, *TC = T[(q1 + 1) & 3] + rC;
             uint16_t *Hw = H1[q1 & 1] + v1 * VS + 4 * c;
 #pragma unroll
             for (int k = 0; k < G1W; ++k) {
-                if (w + FWAVES * k < F1G) {
+                if (FWAVES * (k + 1) <= F1G || w + FWAVES * k < F1G) {
                     const int o16 = 16 * FWAVES * k;
                     const u32x2 ta = TA[o16], tb = TB[o16], tc = TC[o16];
                     u32x4 raw;
@@ -553,41 +637,60 @@ __global__ __launch_bounds__(FTHREADS) __attribute__((amdgpu_waves_per_eu(3, 3))
                     raw[3] = c == 0 ? tc[0] : (c == 1 ? (tc[1] & 0xffffu) : 0u);
                     const f32x4 a = mfma16(A1, __builtin_bit_cast(bf16x8, raw), f32x4{0.f, 0.f, 0.f, 0.f});
                     u32x2 pk = relu_bf16x4(a);
-                    if (!(zin && ((ok1 >> k) & 1u))) pk[0] = pk[1] = 0u;
+                    const uint32_t m1 = (zin && ((ok1 >> k) & 1u)) ? 0xffffffffu : 0u;
+                    pk[0] &= m1;
+                    pk[1] &= m1;
                     *(u32x2 *)(Hw + o16 * VS) = pk;
                 }
             }
         }
-        // layer 3: h2 plane q3 = i-4 feeds output planes q3+1 (row 0), q3 (row 1), q3-1 (row 2, done)
+        // layer 3: h2 plane q3 = i-4 feeds output planes q3+1 (dz = -1, lanes 0-15), q3 (lanes
+        // 16-31) and q3-1 (lanes 32-47, done).  Each lane group takes the running sums of the
+        // group below it, so an output is ((P(-1) + P(0)) + P(+1)); lanes 48-63 hold nothing.
         const int q3 = i - 4;
-        if (q3 >= z0 - 1 && q3 <= z1) {
+        if (STEADY || (q3 >= z0 - 1 && q3 <= z1)) {
             const int o = q3 - 1;
-            const bool store = o >= z0 && o < z1;
-            const uint16_t *Hp = H2[q3 & 1];
+            const uint16_t *Hp = H2[q3 & 1] + b3;
+            u32x4 raw[NSTEP3B];
 #pragma unroll
-            for (int k = 0; k < G3W; ++k) {
-                if (w + FWAVES * k < F3G) {
-                    u32x4 raw[NSTEP3F];
-                    load_pairs(Hp, std::integral_constant<int, F2X>{}, F2V * VS, b3 + FWAVES * k * F2X * VS, raw);
-                    f32x4 d = {0.f, 0.f, 0.f, 0.f};
+            for (int s3 = 0; s3 < NSTEP3B; ++s3) raw[s3] = *(const u32x4 *)(Hp + ((s3 / 3) * F2X + s3 % 3) * VS);
+            f32x4 d = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                    for (int p = 0; p < NSTEP3F; ++p) d = mfma16(A3s[p * 64 + lane], __builtin_bit_cast(bf16x8, raw[p]), d);
-                    acc3[NXT][k] = d[0];
-                    acc3[CUR][k] = acc3[CUR][k] + d[1];
-                    acc3[PRV][k] = acc3[PRV][k] + d[2];
-                    if (store && ((ok3 >> k) & 1u)) out[o * plane + ob + (int64_t)FWAVES * k * nx] = acc3[PRV][k] * oscale;
-                }
+            for (int s3 = 0; s3 < NSTEP3B; ++s3) d = mfma16(A3s[s3 * 64 + lane], __builtin_bit_cast(bf16x8, raw[s3]), d);
+#pragma unroll
+            for (int j = 0; j < L3ROWS; ++j) {
+                const float below = __shfl_up(run3[j], 16, 64);
+                run3[j] = c == 0 ? d[j] : below + d[j];
+            }
+            if (c == 2 && o >= z0 && o < z1) {
+#pragma unroll
+                for (int j = 0; j < L3ROWS; ++j)
+                    if ((ok3 >> j) & 1u) out[o * plane + ob + (int64_t)j * nx] = run3[j] * oscale;
             }
         }
         __syncthreads();
     };
     // steps z0-4 .. z1+4, unrolled by three for the rotating slots
-    const int last = z1 + 4;
-    for (int i = z0 - 4; i <= last; i += 3) {
-        step(Slot<0>{}, i);
-        if (i + 1 <= last) step(Slot<1>{}, i + 1);
-        if (i + 2 <= last) step(Slot<2>{}, i + 2);
+    // steps z0-4 .. z1+4; the rotating slot of step i is (i - z0 + 4) % 3.  Steps z0+3 .. z1+2
+    // run every layer (the steady state, no range tests); the first 7 and last 2 test ranges.
+    using Tr = std::true_type;
+    using Fa = std::false_type;
+    auto any_step = [&](int i, auto steadyc) __attribute__((always_inline)) {
+        switch ((i - z0 + 4) % 3) {
+        case 0: step(Slot<0>{}, i, steadyc); break;
+        case 1: step(Slot<1>{}, i, steadyc); break;
+        default: step(Slot<2>{}, i, steadyc); break;
+        }
+    };
+    int i = z0 - 4;
+    for (; i <= z0 + 2; ++i) any_step(i, Fa{});
+    for (; i + 2 <= z1 + 2; i += 3) {  // slots 1, 2, 0 (z0+3 is slot 1)
+        step(Slot<1>{}, i, Tr{});
+        step(Slot<2>{}, i + 1, Tr{});
+        step(Slot<0>{}, i + 2, Tr{});
     }
+    for (; i <= z1 + 2; ++i) any_step(i, Tr{});
+    for (; i <= z1 + 4; ++i) any_step(i, Fa{});
 }
 
 }  // namespace cnn
