@@ -226,6 +226,39 @@ def measure_roofline(ctx, xs, n, nnz, n1, reps, b_h=None, tol=None):
     return roofline
 
 
+def measure_fixed_iterations(ctx, n1=128, iters=150, steps=3, reps=20):
+    """The reference benchmark's own case on the GPU: generate_matrix at n1^3, b = A 1, x0 = 0,
+    `iters` FP64 CG iterations with tolerance 0 (tree dots), timed on the device as `value`
+    is; plus the SpMV of that matrix timed alone.  Leaves the context's matrix at n1^3."""
+    import numpy as np
+    import hpccg_amd
+    ctx.generate(n1, n1, n1)
+    info = ctx.matrix_info()
+    n, nnz = info["nrow"], info["nnz"]
+    ones = hpccg_amd.DeviceArray.from_numpy(ctx, np.ones(n))
+    b = hpccg_amd.DeviceArray(ctx, n)
+    ctx.spmv_dev(ones, b)  # b = A 1, as generate_matrix forms it (integer row sums, exact)
+    x = hpccg_amd.DeviceArray(ctx, n)
+    ms = []
+    for _ in range(steps + 1):
+        x.upload(np.zeros(n))
+        r = ctx.cg_dev(b, x, max_iter=iters, tolerance=0.0, want_hist=False)
+        ms.append(r["ms"])
+    xerr = float(np.max(np.abs(x.numpy() - 1.0)))
+    y = hpccg_amd.DeviceArray(ctx, n)
+    spmv_ms = ctx.time_spmv(ones, y, reps=reps)
+    fmt = ctx.matrix_format()
+    Bf = spmv_bytes(n, nnz, s_idx=1 if fmt["ncodes"] else 4)
+    B = spmv_bytes(n, nnz)
+    out = {"workload": f"generate_matrix {n1}^3, b = A 1, x0 = 0, {iters} iterations, tolerance 0, FP64",
+           "rows": n, "nnz": nnz, "s_device": round(sum(ms[1:]) / steps / 1e3, 6), "niters": r["niters"],
+           "normr": r["normr"], "xerr_max": xerr, "spmv_avg_ms": round(spmv_ms, 5),
+           "spmv_frac": round(Bf / (spmv_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+           "spmv_effective_frac": round(B / (spmv_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    log(f"{n1}^3 fixed {iters} iterations: {out}")
+    return out
+
+
 def measure_network(ctx, grid, reps):
     """z = net(u) on a random grid^3 input (the fused inference the flexible CG launches):
     30 warm-up applications, then `reps` timed ones (synchronous calls, host clock)."""
@@ -425,6 +458,9 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(grid, b_h, results["fp64_cg"]["niters"], log_dir)
         log(f"cpu baseline: {cpu}")
+    # 128^3, 150 iterations (the reference's fixed-iteration benchmark at that size); replaces
+    # the context's matrix, so it runs last
+    fixed150 = measure_fixed_iterations(ctx, 128, 150, steps=max(1, args.steps))
 
     out = {
         "metric": METRIC,
@@ -448,6 +484,7 @@ def main():
         "fp64_cg_host_arrays": host_arrays,
         "learned_training": train_info,
         "batched": batched,
+        "fp64_cg_128_150it": fixed150,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
