@@ -497,7 +497,7 @@ __global__ __launch_bounds__(kFinishThreads) void kf_true_finish(const double *_
 }
 
 // Ap = A p and the workgroup's share of p.Ap
-__global__ __launch_bounds__(kSellThreads) void kf_spmv(const SellRef<__half> A, const float *__restrict__ p,
+__global__ __launch_bounds__(kSellThreads) __attribute__((amdgpu_waves_per_eu(kSellF16WavesPerEU, kSellF16WavesPerEU))) void kf_spmv(const SellRef<__half> A, const float *__restrict__ p,
                                                         float *__restrict__ Ap, const FcgState *st, double *red_part)
 {
     __shared__ double red[kSellThreads / kWave];

@@ -134,7 +134,7 @@ __global__ __launch_bounds__(kVecThreads) void km_update_p(const float *__restri
 }
 
 // Ap = A p (FP16 values, FP32 p, FP64 accumulation, FP32 store) and the workgroup's p.Ap.
-__global__ __launch_bounds__(kSellThreads) void km_spmv(const SellRef<__half> A, const float *__restrict__ p,
+__global__ __launch_bounds__(kSellThreads) __attribute__((amdgpu_waves_per_eu(kSellF16WavesPerEU, kSellF16WavesPerEU))) void km_spmv(const SellRef<__half> A, const float *__restrict__ p,
                                                         float *__restrict__ Ap, const MixState *st, double *red_part)
 {
     __shared__ double red[kSellThreads / kWave];

@@ -55,6 +55,10 @@ constexpr int kTrueLoop = 4;
 constexpr int kStencilK = 27;         // slice width of the 27-point stencil: fully unrolled path
 constexpr int kGroupF64 = 2;          // FP64 copy: 2 entries per lane per load (16-byte value loads)
 constexpr int kGroupF16 = 1;          // FP16 copy: 1 entry per lane per load (measured best)
+// FP16-copy SpMVs of the CG loops (km_spmv, kf_spmv) are compiled for 7 waves per SIMD (at most
+// 72 VGPRs).  Measured at 256^3: 0.293 ms with the compiler's own choice (80 VGPRs, 6 waves),
+// 0.279 ms with 7, 0.316 ms with 5 and 0.421 ms with 8.  The FP64 SpMV is fastest at its own 5.
+constexpr int kSellF16WavesPerEU = 7;
 constexpr int kVecThreads = 256;
 constexpr int kMaxGrid = 16384;       // partial slots of the vector kernels' grid reductions
 
