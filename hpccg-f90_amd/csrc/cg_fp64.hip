@@ -73,6 +73,14 @@ __global__ __launch_bounds__(kSellThreads) void k_spmv(const SellRef<double> A, 
     sell_spmv_f64(A, x, y, false);
 }
 
+// Prepare's Ap = A x, skipped when k_nonzero found x = 0 (k_cg_prep_r then reads Ap as +0.0)
+__global__ __launch_bounds__(kSellThreads) void k_spmv_prep(const SellRef<double> A, const double *__restrict__ x,
+                                                            double *__restrict__ y, const CgState *st)
+{
+    if (!st->x_nonzero) return;
+    sell_spmv_f64(A, x, y, false);
+}
+
 // ---- CG control logic shared by the tree and sequential variants ---------------------------
 __device__ __forceinline__ bool cg_stopped(const CgState *st)
 {
@@ -132,16 +140,18 @@ __global__ __launch_bounds__(kFinishThreads) void k_cg_pap_finish(const double *
 template <bool TREE>
 __global__ __launch_bounds__(kVecThreads) void k_cg_prep_r(const double *__restrict__ b,
                                                            const double *__restrict__ Ap,
-                                                           double *__restrict__ r, int n, double *partials)
+                                                           double *__restrict__ r, int n, double *partials,
+                                                           const CgState *st)
 {
     __shared__ double red[kVecThreads / kWave];
+    const bool x0 = !st->x_nonzero;  // x = 0: Ap = A x is +0.0 in every row and was not formed
     int64_t i0, i1;
     int lb;
     elem_range(n, i0, i1, lb);
     double dot = 0.0;
     for (int64_t i = i0 + threadIdx.x; i < i1; i += kVecThreads) {
         const double t1 = 1.0 * b[i];
-        const double t2 = -1.0 * Ap[i];
+        const double t2 = -1.0 * (x0 ? 0.0 : Ap[i]);
         const double ri = t1 + t2;
         r[i] = ri;
         dot = fma(ri, ri, dot);
@@ -558,15 +568,19 @@ static int enqueue_iterations(hpccg_ctx *c, const CgWs &w, double *x, int count)
 static int enqueue_prepare(hpccg_ctx *c, const CgWs &w, const double *b, const double *x)
 {
     const int n = c->A.nrow, Gv = vec_grid(c, n);
+    // x = 0 (the usual start) needs no A x: k_nonzero sets the control block's flag otherwise
+    hipLaunchKernelGGL(k_nonzero<>, dim3(Gv), dim3(kVecThreads), 0, c->stream, x, (int64_t)n, &w.state->x_nonzero);
     if (c->A.nslices > 0)
-        hipLaunchKernelGGL(k_spmv, dim3(sell_grid(c->A)), dim3(kSellThreads), 0, c->stream, sell64(c->A), x, w.Ap);
+        hipLaunchKernelGGL(k_spmv_prep, dim3(sell_grid(c->A)), dim3(kSellThreads), 0, c->stream, sell64(c->A), x, w.Ap,
+                           w.state);
     if (c->dot_mode == HPCCG_DOT_TREE) {
-        hipLaunchKernelGGL(k_cg_prep_r<true>, dim3(Gv), dim3(kVecThreads), 0, c->stream, b, w.Ap, w.r, n, w.vec_part);
+        hipLaunchKernelGGL(k_cg_prep_r<true>, dim3(Gv), dim3(kVecThreads), 0, c->stream, b, w.Ap, w.r, n, w.vec_part,
+                           w.state);
         hipLaunchKernelGGL(k_cg_rr_finish<true>, dim3(1), dim3(kFinishThreads), 0, c->stream, w.vec_part, Gv, w.state,
                            w.rr_hist);
     } else {
         hipLaunchKernelGGL(k_cg_prep_r<false>, dim3(Gv), dim3(kVecThreads), 0, c->stream, b, w.Ap, w.r, n,
-                           w.vec_part);
+                           w.vec_part, w.state);
         hipLaunchKernelGGL(k_cg_serial_rr, dim3(1), dim3(kWave), 0, c->stream, w.r, n, w.state, w.rr_hist, 1);
     }
     HPCCG_HIP_TRY(hipGetLastError());

@@ -434,6 +434,7 @@ struct FcgState {
     double alpha, beta, rz, pap, tol, rr;
     double rr_true;
     int k, stop_at, max_iter, replace_every, need_true, converged, breakdown, done;
+    int x_nonzero;      // set by Prepare's k_nonzero when x0 != 0 (else A x0 is skipped)
     float s_in, s_out;  // network input / output scales (1/rms(r), rms(r))
 };
 
@@ -451,13 +452,15 @@ __global__ __launch_bounds__(kTrueThreads) void kf_true(const SellRef<__half> A,
 {
     __shared__ double red[kTrueThreads / kWave];
     if (!kf_true_due<PREP>(st)) return;
+    const bool x0 = PREP && !st->x_nonzero;  // x = 0: A x is +0.0 in every row
     double dot = 0.0;
     for (int rep = 0; rep < kTrueLoop; ++rep) {
         int64_t row;
         bool valid;
-        const double ax = sell_rowsum_of_thread<kGroupF16, __half, double, kTrueWaves>(A, x, row, valid,
-                                                                                      blockIdx.x * kTrueLoop + rep,
-                                                                                      rep == 0);
+        const int g = blockIdx.x * kTrueLoop + rep;
+        double ax = 0.0;
+        if (x0) sell_row_of_thread<kTrueWaves>(A.nslices, A.nrow, g, row, valid);
+        else ax = sell_rowsum_of_thread<kGroupF16, __half, double, kTrueWaves>(A, x, row, valid, g, rep == 0);
         if (valid) {
             const float ri = (float)(b[row] - ax);
             __builtin_nontemporal_store(ri, r + row);
@@ -1136,6 +1139,8 @@ int hpccg_fcg_learned_dev(hpccg_ctx *c, int32_t nx, int32_t ny, int32_t nz, cons
 
     HPCCG_HIP_TRY(hipEventRecord(c->ev2, c->stream));
     // Prepare: r = b - A x, z = M(r), rz = z.r, p = z
+    // x = 0 (the usual start) needs no A x: the flag is set otherwise
+    hipLaunchKernelGGL(k_nonzero<>, dim3(Gv), dim3(kVecThreads), 0, c->stream, x, (int64_t)n, &F->st->x_nonzero);
     hipLaunchKernelGGL(kf_true<true>, dim3(Gt), dim3(kTrueThreads), 0, c->stream, sell16(A), x, b, F->r, F->st,
                        A.red_part);
     hipLaunchKernelGGL(kf_true_finish<true>, dim3(1), dim3(kFinishThreads), 0, c->stream, A.red_part, Gt, n, F->st,

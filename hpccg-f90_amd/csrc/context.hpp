@@ -24,6 +24,7 @@ struct CgState {
     int stop_at;       // first iteration that does not run (max_iter+1 or lagged test)
     int max_iter;
     int breakdown;     // p.Ap was zero or non-finite
+    int x_nonzero;     // set by Prepare's k_nonzero when x0 != 0 (else A x0 is skipped)
     double rr;         // ||r_{k-1}||^2, the last recorded r.r
 };
 

@@ -39,6 +39,7 @@ struct MixState {
     int converged;
     int breakdown;
     int inexact16;
+    int x_nonzero;  // set by Prepare's k_nonzero when x0 != 0 (else A x0 is skipped)
 };
 
 // ---- kernels --------------------------------------------------------------------------------
@@ -58,13 +59,15 @@ __global__ __launch_bounds__(kTrueThreads) void km_true_residual(const SellRef<_
 {
     __shared__ double red[kTrueThreads / kWave];
     if (!km_true_due<PREP>(st)) return;
+    const bool x0 = PREP && !st->x_nonzero;  // x = 0: A x is +0.0 in every row
     double dot = 0.0;
     for (int rep = 0; rep < kTrueLoop; ++rep) {
         int64_t row;
         bool valid;
-        const double ax = sell_rowsum_of_thread<kGroupF16, __half, double, kTrueWaves>(A, x, row, valid,
-                                                                                      blockIdx.x * kTrueLoop + rep,
-                                                                                      rep == 0);
+        const int g = blockIdx.x * kTrueLoop + rep;
+        double ax = 0.0;
+        if (x0) sell_row_of_thread<kTrueWaves>(A.nslices, A.nrow, g, row, valid);
+        else ax = sell_rowsum_of_thread<kGroupF16, __half, double, kTrueWaves>(A, x, row, valid, g, rep == 0);
         if (valid) {
             const float ri = (float)(b[row] - ax);
             __builtin_nontemporal_store(ri, r + row);
@@ -318,6 +321,9 @@ static void launch_km_true(hpccg_ctx *c, const double *b, const double *x)
     const int G = true_blocks(A.nslices);
     MixedWs &m = c->mix;
     MixState *st = (MixState *)m.state;
+    if (PREP)  // x = 0 (the usual start) needs no A x: the flag is set otherwise
+        hipLaunchKernelGGL(k_nonzero<>, dim3(vec_grid_for(c->info.num_cu, A.nrow)), dim3(kVecThreads), 0, c->stream, x,
+                           (int64_t)A.nrow, &st->x_nonzero);
     hipLaunchKernelGGL(km_true_residual<PREP>, dim3(G), dim3(kTrueThreads), 0, c->stream, sell16(A), x, b, m.r, st,
                        A.red_part);
     hipLaunchKernelGGL(km_true_finish<PREP>, dim3(1), dim3(kFinishThreads), 0, c->stream, A.red_part, G, st,

@@ -278,6 +278,30 @@ __device__ __forceinline__ int sell_block()
     return (b / span) * span + (b % kNumXcd) * kXcdRun + (b / kNumXcd) % kXcdRun;
 }
 
+// The row of this thread in slice group g of a one-wave-per-slice launch (WAVES slices per
+// group), whether it exists, and its slice.
+template <int WAVES>
+__device__ __forceinline__ int sell_row_of_thread(int nslices, int nrow, int g, int64_t &row, bool &valid)
+{
+    const int s = g * WAVES + __builtin_amdgcn_readfirstlane(threadIdx.x / kSlice);
+    row = (int64_t)s * kSlice + threadIdx.x % kSlice;
+    valid = s < nslices && row < nrow;
+    return s;
+}
+
+// Sets *flag when x[0..n) holds an entry other than +-0.0 (a NaN counts).  The flag lives in a
+// solver's control block, which the host zeroes before Prepare.  A solve that starts from
+// x = 0 then skips Prepare's A x: every row sum of A 0 is +0.0 (products +-0 added to 0.0), so
+// r = b - A x is formed from b alone with the same bits.
+template <typename Unused = void>
+__global__ __launch_bounds__(kVecThreads) void k_nonzero(const double *__restrict__ x, int64_t n, int *flag)
+{
+    bool nz = false;
+    for (int64_t i = blockIdx.x * (int64_t)kVecThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kVecThreads)
+        nz |= !(x[i] == 0.0);
+    if (__any(nz) && threadIdx.x % kWave == 0) atomicOr(flag, 1);
+}
+
 // XRUN: number the workgroups with sell_block().  Used by the FP64 SpMV and the mixed-precision
 // FP16 SpMV (coded copy at 256^3: mixed CG 36.0 -> 35.6 ms); the flexible-CG SpMV, which reads
 // the preconditioner's output just written, keeps hardware order (22.2 -> 22.4 ms with it).
@@ -294,10 +318,8 @@ __device__ __forceinline__ double sell_rowsum_of_thread(const SellRef<VT> &A, co
     }
     // slice group: `group` when given, else this workgroup's (renumbered when XRUN)
     const int g = group >= 0 ? group : XRUN ? sell_block() : (int)blockIdx.x;
-    const int s = g * WAVES + __builtin_amdgcn_readfirstlane(threadIdx.x / kSlice);
+    const int s = sell_row_of_thread<WAVES>(A.nslices, A.nrow, g, row, valid);
     const int lane = threadIdx.x % kSlice;
-    row = (int64_t)s * kSlice + lane;
-    valid = s < A.nslices && row < A.nrow;
     if (s >= A.nslices) return 0.0;
     if (A.code)
         return sell_rowsum_coded<G, VT, XT>(A.slice_ptr, A.code, [&](int c) { return tab[c]; }, A.val, A.rowptr, x, s,
