@@ -66,7 +66,8 @@ struct Batch {
     float *r = nullptr, *p = nullptr, *Ap = nullptr;
     BatchState *st = nullptr;
     double *partials = nullptr;         // [2 NS]: per-slice dot partials
-    int *nactive = nullptr;             // systems still iterating (0: every launch returns at once)
+    int *nactive = nullptr;             // [0] systems still iterating (0: every launch returns at once),
+                                        // [1] Prepare's k_nonzero flag (x0 != 0)
     int best_variant = -1;
     BatchState *snap = nullptr;         // pinned: [2][S] state snapshots (after Prepare, latest)
     int hint = 0;                       // iterations the previous solve needed (0: none yet)
@@ -200,12 +201,13 @@ __global__ __launch_bounds__(W * 64) void kb_true(const BatchSell A, const doubl
                                                    const BatchState *st, double *part, const int *nactive)
 {
     if (!PREP && *nactive == 0) return;
+    const bool x0 = PREP && nactive[1] == 0;  // x = 0: A x is +0.0 in every row
     const int lane = threadIdx.x % kSlice;
     wave_slices<W, SPW>(A.NS, [&](int gs) {
         const SlicePos q = slice_pos(A, gs, lane);
         const BatchState *S = st + q.s;
         if (!PREP && (!S->active || !S->need_true)) return;
-        const double ax = batch_rowsum(A, q, x, gs, lane);
+        const double ax = x0 ? 0.0 : batch_rowsum(A, q, x, gs, lane);
         double rr = 0.0, bb = 0.0;
         if (q.valid) {
             const double bi = b[q.gr];
@@ -407,6 +409,9 @@ static void enqueue_batch(hpccg_ctx *c, Batch *B, const double *b, double *x, in
     double *part = B->partials;
     int *na = B->nactive;
     if (iters < 0) {
+        // x = 0 (the usual start) needs no A x: the flag is set otherwise
+        hipLaunchKernelGGL(k_nonzero<>, dim3(vec_grid_for(c->info.num_cu, B->E)), dim3(kVecThreads), 0, c->stream, x,
+                           B->E, na + 1);
         hipLaunchKernelGGL((kb_true<W, SPW, true>), grid, blk, 0, c->stream, A, x, b, B->r, B->st, part, na);
         hipLaunchKernelGGL(kb_true_finish<true>, sys, fin, 0, c->stream, part, B->slice_off, B->st, rel_tol2, na);
         return;
@@ -465,8 +470,8 @@ static int batch_solve(hpccg_ctx *c, Batch *B, const double *b, double *x, doubl
         s.max_iter = max_iter;
     }
     HPCCG_HIP_TRY(hipMemcpyAsync(B->st, init.data(), S * sizeof(BatchState), hipMemcpyHostToDevice, c->stream));
-    static const int zero = 0;
-    HPCCG_HIP_TRY(hipMemcpyAsync(B->nactive, &zero, sizeof(int), hipMemcpyHostToDevice, c->stream));
+    static const int zero[2] = {0, 0};  // active count, x0 flag
+    HPCCG_HIP_TRY(hipMemcpyAsync(B->nactive, zero, sizeof(zero), hipMemcpyHostToDevice, c->stream));
     const double rel_tol2 = rel_tol * rel_tol;
     // iteration graphs bake b, x and the variant (tolerances and limits live in the states)
     auto get = [&](int size, hipGraphExec_t *ge) {
@@ -584,7 +589,7 @@ int hpccg_batch_setup(hpccg_ctx *c, int32_t count, const int32_t *dims)
         for (int i = soff[s]; i < soff[s + 1]; ++i) ssys[i] = s;
     if (dmalloc(&B->slice_sys, ssys.size() * sizeof(int)) != hipSuccess ||
         dmalloc(&B->partials, 2 * ssys.size() * sizeof(double)) != hipSuccess ||
-        dmalloc(&B->nactive, sizeof(int)) != hipSuccess ||
+        dmalloc(&B->nactive, 2 * sizeof(int)) != hipSuccess ||
         hipHostMalloc(&B->snap, 2 * (size_t)count * sizeof(BatchState), hipHostMallocDefault) != hipSuccess) {
         free_batched(c);
         set_error("batched: device allocation failed");
