@@ -330,6 +330,12 @@ def test_cg_edge_cases(ctx):
             assert np.array_equal(x, xr) and hist.tolist() == hr.tolist()
         else:
             assert np.max(np.abs(x - xr)) < 1e-10
+        # a start of negative zeros: Prepare skips A x as for +0 and the reference, which forms
+        # A x, gets the same r = b (every row sum of A (-0) is +0.0)
+        xr0, _, _, hr0, _ = oracle.cg(A, A.b, x0=np.full(A.n, -0.0), max_iter=20)
+        x, nit, normr, hist, st = ctx.cg(A.b, x0=np.full(A.n, -0.0), max_iter=20)
+        if mode == hpccg_amd.DOT_SEQUENTIAL:
+            assert np.array_equal(x, xr0) and hist.tolist() == hr0.tolist()
         # breakdown: b = 0 gives p.Ap = 0 at iteration 1 -> NaN, reported as a warning
         x, nit, normr, hist, st = ctx.cg(np.zeros(A.n), max_iter=3)
         assert st == hpccg_amd._lib.WARN_BREAKDOWN

@@ -160,3 +160,24 @@ def test_mixed_cg_over_int32_columns(ctx):
     assert st == 0
     rel = math.sqrt(oracle.ddot1(b - oracle.spmv(A, x))) / normb
     assert rel <= 1e-10, rel
+
+
+def test_cg_mixed_from_a_nonzero_start(ctx):
+    """Prepare forms b - A x0 with the SpMV when x0 != 0 (it skips it only for x0 = 0): its true
+    residual is the oracle's ||b - A x0|| in FP32 steps, and the solve still meets 1e-10."""
+    A = oracle.Problem(24, 20, 16)
+    xs = oracle.xstar_sines(24, 20, 16, 8)
+    b = oracle.spmv(A, xs)
+    normb = math.sqrt(oracle.ddot1(b))
+    ctx.upload(A.rowindex, A.colindex, A.values)
+    x0 = oracle.xstar_uniform(A.n, 31)
+    r0 = (b - oracle.spmv(A, x0)).astype(np.float32).astype(np.float64)
+    x, nit, tn, hr, ht, st = ctx.cg_mixed(b, x0=x0, max_iter=1000, tolerance=1e-10 * normb, replace_every=25)
+    assert st == 0
+    assert ht[0] == pytest.approx(math.sqrt(float(r0 @ r0)), rel=1e-12)
+    r = b - oracle.spmv(A, x)
+    assert math.sqrt(oracle.ddot1(r)) / normb <= 1e-10
+    # a start of negative zeros is a zero start: the same iterates as from +0
+    xz, nz, _, hz, _, _ = ctx.cg_mixed(b, x0=np.full(A.n, -0.0), max_iter=40, tolerance=0.0)
+    xp, npos, _, hp, _, _ = ctx.cg_mixed(b, max_iter=40, tolerance=0.0)
+    assert nz == npos and np.array_equal(xz, xp) and np.array_equal(hz, hp)
