@@ -296,7 +296,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the z-slab sharding design is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+              "(a multi-GPU run would shard the grid into z-slabs, one per GPU); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -305,7 +305,7 @@ def main():
 
     n1 = args.grid
     grid = (n1, n1, n1)
-    log_dir = os.path.join(ROOT, "gpurun_out")
+    log_dir = os.environ.get("HPCCG_BENCH_LOGDIR", os.path.join(ROOT, "bench_out"))
     os.makedirs(log_dir, exist_ok=True)
 
     ctx = hpccg_amd.Context(0, hpccg_amd.DOT_TREE)

@@ -1,7 +1,7 @@
 // Learned preconditioner: network state, training on residual/error pairs that plain CG
 // generates, and flexible CG with the network as preconditioner.  No reference counterpart;
 // it extends cg (MPI/src/numerics.F90:41-162), whose FP64 path is both its fallback and the
-// judge of its answers (FP64 true residual).
+// measure of its answers (FP64 true residual).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

@@ -1,6 +1,6 @@
 """GPU tests of the batched solve (64 uneven 27-point systems in one graph launch).
 
-Each system is a generate_matrix grid; the oracle judges each independently: the batched
+Each system is a generate_matrix grid; the oracle checks each independently: the batched
 SpMV equals the oracle's per-system FP64 spmv rounded once to FP32 (bitwise), and every
 system's answer meets rel_tol in the FP64 true residual computed by the oracle."""
 import math

@@ -160,7 +160,7 @@ def _system(n, seed):
 def test_fcg_learned_gain_and_accuracy(ctx):
     """The shipped network (hpccg_amd/data, trained on 64^3 pairs and selected on a 128^3
     validation system) at least halves flexible-CG iterations against plain CG's on a fresh 32^3
-    system, and the answer meets the tolerance in the FP64 true residual (judged by the oracle)."""
+    system, and the answer meets the tolerance in the FP64 true residual (checked by the oracle)."""
     from hpccg_amd import learned
     w = learned.load()
     assert w is not None and w.size == oracle.NW1 + oracle.NW2 + oracle.NW3

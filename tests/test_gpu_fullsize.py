@@ -1,4 +1,4 @@
-"""Full-size GPU parity: the benchmark workloads at their own sizes, every answer judged by the
+"""Full-size GPU parity: the benchmark workloads at their own sizes, every answer checked by the
 FP64 oracle (oracle.spmv forms the true residual b - A x on the CPU) or by the reference's own
 fixtures (tests/golden/cg_solve_sines.json: the reference cg solving the same seeded systems).
 

@@ -1,5 +1,5 @@
 """GPU tests of the mixed-precision CG (FP16 values, FP32 vectors, FP64 dots / x, residual
-replacement).  It has no reference counterpart; it is judged by the FP64 oracle: the true
+replacement).  It has no reference counterpart; it is checked by the FP64 oracle: the true
 residual ||b - A x|| / ||b|| of its answer (computed by the oracle in FP64) must meet the
 tolerance, and its error against x* must be as small as the FP64 reference solve's."""
 import math

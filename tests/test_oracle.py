@@ -63,7 +63,7 @@ def test_closed_forms(dims):
 
 
 def test_cube_sizes_table():
-    # the survey's table, 32^3: nnz 830,584, ||b||^2 = 662,360
+    # closed forms at 32^3: nnz = (3n-2)^3 = 830,584, ||b||^2 = 662,360
     A = oracle.Problem(32, 32, 32)
     assert A.nnz == 830584
     assert oracle.ddot1(A.b) == 662360.0
