@@ -178,7 +178,67 @@ def cpu_baseline(grid, b, gpu_niters, log_dir):
             "host": host, "plan": plan}
 
 
-def measure_roofline(ctx, xs, n, nnz, n1, reps, b_h=None, tol=None):
+def counter_kib(csv_dir, counter):
+    """Per-kernel mean of a rocprofv3 counter (KiB per dispatch) from its CSV output."""
+    import csv
+    import glob
+    vals = {}
+    for path in glob.glob(os.path.join(csv_dir, "**", "*counter_collection.csv"), recursive=True):
+        with open(path) as f:
+            for row in csv.DictReader(f):
+                if row.get("Counter_Name") != counter:
+                    continue
+                name = row.get("Kernel_Name", "")
+                key = "spmv" if "k_cg_spmv<true>" in name else ("copy" if "k_copy16" in name else None)
+                if key:
+                    vals.setdefault(key, []).append(float(row["Counter_Value"]))
+    return {k: (sum(v) / len(v), len(v)) for k, v in vals.items()}
+
+
+def measure_traffic(n1, timeout=180):
+    """HBM bytes of one k_cg_spmv<true> launch at n1^3, measured now: FETCH_SIZE and WRITE_SIZE
+    each from a rocprofv3 --pmc run of its own (a child process running `bench.py --spmv-only`,
+    started before this process uses the GPU), each scaled by what the same counter reads for the
+    copy kernel of known size that the same run times.  None when the profiler is absent or a
+    run fails (the caller then reports the stored measurement)."""
+    exe = shutil.which("rocprofv3")
+    if not exe or os.environ.get("HPCCG_BENCH_NO_COUNTERS"):
+        return None
+    # the copy of that run moves min(B / 2, 2 GiB) rounded down to 16 bytes each way
+    n = n1 ** 3
+    cb = min(spmv_bytes(n, (3 * n1 - 2) ** 3) // 2, 2 ** 31) // 16 * 16
+    out = {"kernel": "k_cg_spmv<true>", "grid": n1, "copy_bytes_each_way": cb}
+    for ctr, key in (("FETCH_SIZE", "fetch"), ("WRITE_SIZE", "write")):
+        d = tempfile.mkdtemp(prefix="hpccg_pmc_")
+        cmd = [exe, "--pmc", ctr, "--output-format", "csv", "-d", d, "-o", "run", "--", sys.executable,
+               os.path.abspath(__file__), "--spmv-only", "--spmv-reps", "5", "--grid", str(n1)]
+        p = subprocess.Popen(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, cwd=tempfile.gettempdir(),
+                             start_new_session=True)
+        try:
+            p.wait(timeout=timeout)
+        except subprocess.TimeoutExpired:
+            os.killpg(p.pid, 9)
+            p.wait()
+            log(f"counter run {ctr} timed out")
+            return None
+        kib = counter_kib(d, ctr)
+        shutil.rmtree(d, ignore_errors=True)
+        if p.returncode != 0 or "spmv" not in kib or "copy" not in kib or kib["copy"][0] <= 0:
+            log(f"counter run {ctr}: rc={p.returncode}, kernels {sorted(kib)}")
+            return None
+        scale = cb / (kib["copy"][0] * 1024.0)
+        out[f"{key}_kib_per_dispatch"] = round(kib["spmv"][0], 1)
+        out[f"{key}_dispatches"] = kib["spmv"][1]
+        out[f"copy_{key}_kib"] = round(kib["copy"][0], 1)
+        out[f"{key}_scale"] = round(scale, 6)
+        out[f"{key}_bytes"] = int(round(kib["spmv"][0] * 1024.0 * scale))
+        out[f"{key}_command"] = " ".join(["rocprofv3", "--pmc", ctr, "--", "python3", "bench.py", "--spmv-only",
+                                          "--spmv-reps", "5", "--grid", str(n1)])
+    out["traffic_bytes"] = out["fetch_bytes"] + out["write_bytes"]
+    return out
+
+
+def measure_roofline(ctx, xs, n, nnz, n1, reps, b_h=None, tol=None, live_traffic=None):
     """Roofline of the dominant kernel: the FP64 SpMV as the CG loop launches it, timed with HIP
     events on the context's stream, against the bytes its device format must move (`frac`); the
     CSR formula's bytes give `effective_frac`.  With b_h, also the same kernel inside a CG solve
@@ -195,11 +255,15 @@ def measure_roofline(ctx, xs, n, nnz, n1, reps, b_h=None, tol=None):
     achieved = Bf / (spmv_ms * 1e-3) / 1e9
     traffic, tsrc = None, None
     tf = os.path.join(ROOT, "profiles", f"spmv_fp64_{n1}_traffic.json")
-    if os.path.exists(tf):
+    if live_traffic:
+        traffic = live_traffic["traffic_bytes"]
+        tsrc = dict(live_traffic, measured="in this run: two rocprofv3 --pmc child runs before the timed solves")
+    elif os.path.exists(tf):
         with open(tf) as f:
             t = json.load(f)
         traffic = t.get("traffic_bytes")
         tsrc = {k: t.get(k) for k in ("measured_at", "fetch_scale", "write_scale", "source")}
+        tsrc["measured"] = "stored measurement (the counter runs of this run were unavailable)"
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "k_cg_spmv<true> (FP64 sliced-ELL SpMV with one-byte column codes + p.Ap partials)",
@@ -298,6 +362,15 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(a multi-GPU run would shard the grid into z-slabs, one per GPU); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    # HBM traffic of the SpMV from hardware counters, in child processes that run before this
+    # one uses the GPU
+    live_traffic = None
+    if not (args.spmv_only or args.network_only):
+        t_ctr = time.perf_counter()
+        live_traffic = measure_traffic(args.grid)
+        log(f"counter runs: {time.perf_counter() - t_ctr:.1f} s, traffic "
+            f"{live_traffic['traffic_bytes'] if live_traffic else None}")
 
     import numpy as np
     import hpccg_amd
@@ -419,7 +492,8 @@ def main():
     for k in results:
         results[k]["speedup_vs_fp64_cg"] = round(results["fp64_cg"]["s_wall"] / results[k]["s_wall"], 3)
 
-    roofline = measure_roofline(ctx, xs, n, nnz, n1, args.spmv_reps, b_h=b_h, tol=REL_TOL * normb)
+    roofline = measure_roofline(ctx, xs, n, nnz, n1, args.spmv_reps, b_h=b_h, tol=REL_TOL * normb,
+                                live_traffic=live_traffic)
 
     # batched solves (64 systems, extents ~ U{16..96}, x* ~ U(-1,1), to 1e-8 each; the solve
     # enqueues graph chunks sized from the batch's residual rate, and a repeated solve the count
