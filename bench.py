@@ -324,22 +324,26 @@ def measure_fixed_iterations(ctx, n1=128, iters=150, steps=3, reps=20):
 
 
 def measure_network(ctx, grid, reps):
-    """z = net(u) on a random grid^3 input (the fused inference the flexible CG launches):
-    30 warm-up applications, then `reps` timed ones (synchronous calls, host clock)."""
+    """Roofline of the flexible CG's dominant kernel, the fused network inference z = net(u)
+    (cnn::k_fused_fwd), on a random grid^3 input: `reps` back-to-back applications timed with
+    HIP events on the context's stream (the stream it is launched on), against the 2.5 PFLOP/s
+    dense BF16 MFMA peak.  Algorithmic work: 2 * 27 * (16 + 16*16 + 16) FLOP per voxel; HBM:
+    the FP32 input and output, 8 bytes per voxel."""
     import numpy as np
     import hpccg_amd
     nvox = grid[0] * grid[1] * grid[2]
     u = hpccg_amd.DeviceArray.from_numpy(ctx, np.random.default_rng(1).standard_normal(nvox).astype(np.float32))
     z = hpccg_amd.DeviceArray(ctx, nvox, np.float32)
-    for _ in range(30):
+    for _ in range(10):
         ctx.cnn_apply_dev(grid, u, 1.0, z)
-    t = time.perf_counter()
-    for _ in range(reps):
-        ctx.cnn_apply_dev(grid, u, 1.0, z)
-    ms = (time.perf_counter() - t) / reps * 1e3
+    ms = ctx.time_cnn_apply(grid, u, z, reps=reps)
     flop = 2.0 * 27 * (16 + 16 * 16 + 16) * nvox
-    return {"grid": list(grid), "ms_per_application": round(ms, 4), "tflops": round(flop / (ms * 1e-3) / 1e12, 1),
-            "bf16_peak_tflops": 2500.0}
+    tflops = flop / (ms * 1e-3) / 1e12
+    return {"bound": "mfma", "achieved": round(tflops, 1), "peak": 2500.0, "unit": "TFLOP/s",
+            "frac": round(tflops / 2500.0, 4), "traffic": None, "kernel": "cnn::k_fused_fwd (3-layer BF16 CNN, "
+            "MFMA 16x16x32, FP32 accumulation)", "grid": list(grid), "flop_per_launch": flop,
+            "hbm_bytes_per_launch": 8 * nvox, "avg_ms": round(ms, 5), "ms_per_application": round(ms, 4),
+            "launches": reps}
 
 
 def main():
@@ -534,6 +538,12 @@ def main():
         log(f"cpu baseline: {cpu}")
     # 128^3, 150 iterations (the reference's fixed-iteration benchmark at that size); replaces
     # the context's matrix, so it runs last
+    # the flexible CG's dominant kernel: the network (its weights are set above)
+    roofline_net = None
+    if weights is not None:
+        roofline_net = measure_network(ctx, grid, args.spmv_reps)
+        log(f"network: {roofline_net['avg_ms']} ms per application, {roofline_net['achieved']} TFLOP/s "
+            f"({100 * roofline_net['frac']:.1f}% of BF16 peak)")
     fixed150 = measure_fixed_iterations(ctx, 128, 150, steps=max(1, args.steps))
 
     out = {
@@ -552,6 +562,7 @@ def main():
         "config": {"workload": f"cg-{n1}^3-sines-seed{SEED}-to-relres-1e-10", "grid": list(grid),
                    "solver": headline, "rows": n, "nnz": nnz},
         "roofline": roofline,
+        "roofline_network": roofline_net,
         "cpu_baseline": cpu,
         "setup": setup,
         "solvers": results,

@@ -1034,6 +1034,32 @@ int hpccg_cnn_apply_dev(hpccg_ctx *c, int32_t nx, int32_t ny, int32_t nz, const 
     return 0;
 }
 
+int hpccg_time_cnn_apply(hpccg_ctx *c, int32_t nx, int32_t ny, int32_t nz, const float *r, float *z, int32_t reps,
+                         double *avg_ms)
+{
+    HPCCG_CHECK_ARG(c && r && z && reps > 0 && avg_ms && nx >= 1 && ny >= 1 && nz >= 1, "bad argument");
+    CnnNet *N = net_of(c);
+    if (!N) {
+        set_error("no network");
+        return HPCCG_ERR_STATE;
+    }
+    const Grid3 g = make_grid(nx, ny, nz);
+    HPCCG_TRY(ensure_act(c, g.nvox(), 1));
+    const float sc[2] = {1.0f, 1.0f};
+    HPCCG_HIP_TRY(hipMemcpyAsync(N->scale_in, &sc[0], 4, hipMemcpyHostToDevice, c->stream));
+    HPCCG_HIP_TRY(hipMemcpyAsync(N->scale_out, &sc[1], 4, hipMemcpyHostToDevice, c->stream));
+    forward_fused(c, r, N->scale_in, z, N->scale_out, g);  // warm
+    HPCCG_HIP_TRY(hipEventRecord(c->ev0, c->stream));
+    for (int i = 0; i < reps; ++i) forward_fused(c, r, N->scale_in, z, N->scale_out, g);
+    HPCCG_HIP_TRY(hipGetLastError());
+    HPCCG_HIP_TRY(hipEventRecord(c->ev1, c->stream));
+    HPCCG_HIP_TRY(hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    HPCCG_HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    *avg_ms = ms / reps;
+    return 0;
+}
+
 // Flexible CG preconditioned by the network (mixed precision as hpccg_cg_mixed_dev: FP16 values,
 // FP32 r/p/Ap/z, FP64 dots and x, true-residual replacement every replace_every iterations and
 // a TRUE-residual stopping test).  Every `window` iterations the host compares ||r|| with its

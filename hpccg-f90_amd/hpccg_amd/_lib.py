@@ -87,6 +87,7 @@ _SIGS = {
     "hpccg_batch_dims": (ctypes.c_int, [I32, I32, I32, ctypes.c_uint64, P]),
     "hpccg_time_spmv": (ctypes.c_int, [P, P, P, I32, PD]),
     "hpccg_time_copy": (ctypes.c_int, [P, P, P, I64, I32, PD]),
+    "hpccg_time_cnn_apply": (ctypes.c_int, [P, I32, I32, I32, P, P, I32, PD]),
 }
 
 _lib = None
@@ -515,6 +516,12 @@ class Context:
     def time_spmv(self, x: DeviceArray, y: DeviceArray, reps=20):
         ms = ctypes.c_double()
         _check(lib().hpccg_time_spmv(self.h, x.ptr, y.ptr, reps, ctypes.byref(ms)), "time_spmv")
+        return ms.value
+
+    def time_cnn_apply(self, dims, r: DeviceArray, z: DeviceArray, reps=20):
+        """Mean time in ms of the fused network inference on one field (HIP events)."""
+        ms = ctypes.c_double()
+        _check(lib().hpccg_time_cnn_apply(self.h, *dims, r.ptr, z.ptr, reps, ctypes.byref(ms)), "time_cnn_apply")
         return ms.value
 
     def time_copy(self, dst: DeviceArray, src: DeviceArray, reps=20):

@@ -263,6 +263,12 @@ int hpccg_time_spmv(hpccg_ctx *ctx, const double *x, double *y, int32_t reps, do
 /* A plain device copy of `bytes` bytes, `reps` times: the HBM calibration kernel. */
 int hpccg_time_copy(hpccg_ctx *ctx, void *dst, const void *src, int64_t bytes, int32_t reps,
                     double *avg_ms);
+/* No reference counterpart (the learned preconditioner is an addition): run `reps` back-to-back
+ * applications of the fused network inference z = net(r) (hpccg_cnn_apply_dev with scale 1) on
+ * device arrays of one nx*ny*nz field and return the mean kernel time in ms (HIP events on the
+ * context's stream, the stream the kernel is launched on). */
+int hpccg_time_cnn_apply(hpccg_ctx *ctx, int32_t nx, int32_t ny, int32_t nz, const float *r, float *z,
+                         int32_t reps, double *avg_ms);
 
 #ifdef __cplusplus
 }
