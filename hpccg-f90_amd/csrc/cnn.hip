@@ -1239,7 +1239,7 @@ int hpccg_fcg_learned_dev(hpccg_ctx *c, int32_t nx, int32_t ny, int32_t nz, cons
     if (hist_rec) {
         std::vector<double> h(F->hist_cap);
         HPCCG_HIP_TRY(hipMemcpy(h.data(), hrec, F->hist_cap * 8, hipMemcpyDeviceToHost));
-        for (int j = 0; j <= max_iter; ++j) hist_rec[j] = h[j] >= 0.0 ? std::sqrt(h[j]) : -1.0;
+        for (int j = 0; j <= max_iter; ++j) hist_rec[j] = h[j] != -1.0 ? std::sqrt(h[j]) : -1.0;  // -1: not run
     }
     return 0;
 }

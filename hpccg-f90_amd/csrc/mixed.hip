@@ -417,11 +417,14 @@ static int cg_mixed_device(hpccg_ctx *c, const double *b, double *x, int max_ite
     // the last true residual formed: at the last iteration run (max_iter and convergence both
     // force one), or Prepare's
     const double *ht = h.data() + m.hist_cap;
-    if (true_normr) *true_normr = ht[std::max(0, nit)] >= 0.0 ? std::sqrt(ht[std::max(0, nit)]) : -1.0;
+    // -1 marks an iteration that formed no true residual; a formed one is reported even when it
+    // is NaN (a NaN or Inf in b or x)
+    auto formed = [](double v) { return v != -1.0; };
+    if (true_normr) *true_normr = formed(ht[std::max(0, nit)]) ? std::sqrt(ht[std::max(0, nit)]) : -1.0;
     if (hist_rec)
         for (int j = 0; j <= max_iter; ++j) hist_rec[j] = (j >= 1 && j <= nit) ? std::sqrt(h[j]) : -1.0;
     if (hist_true)
-        for (int j = 0; j <= max_iter; ++j) hist_true[j] = ht[j] >= 0.0 ? std::sqrt(ht[j]) : -1.0;
+        for (int j = 0; j <= max_iter; ++j) hist_true[j] = formed(ht[j]) ? std::sqrt(ht[j]) : -1.0;
     return s.breakdown ? HPCCG_WARN_BREAKDOWN : 0;
 }
 
