@@ -1017,7 +1017,7 @@ int hpccg_cnn_train(hpccg_ctx *c, int32_t steps, int32_t batch, float lr, uint64
 // z = rms(r) * net(r / rms(r)) done inside the kernels (scale = 1/rms(r) given by the caller).
 int hpccg_cnn_apply_dev(hpccg_ctx *c, int32_t nx, int32_t ny, int32_t nz, const float *r, float inv_scale, float *z)
 {
-    HPCCG_CHECK_ARG(c && r && z && inv_scale > 0.f, "bad argument");
+    HPCCG_CHECK_ARG(c && r && z && inv_scale > 0.f && nx >= 1 && ny >= 1 && nz >= 1, "bad argument");
     CnnNet *N = net_of(c);
     if (!N) {
         set_error("no network");
