@@ -414,20 +414,28 @@ __global__ __launch_bounds__(256) void k_wgrad(const uint16_t *__restrict__ dout
 // output voxels and a range of z-planes and marches along z, one plane per step.  In step i every
 // wave does its share of
 //   layer 1: h1 plane i-1 from the input window (planes i-2 .. i; one MFMA per 16 voxels,
-//            K = 27 taps; 25 groups, 3 per wave and a 25th on wave 0);
+//            K = 27 taps; 25 groups);
 //   layer 2: reads h1 plane i-2 ONCE -- five operands per 16 voxels, one per in-plane tap
 //            pair -- and adds it to the three h2 planes it touches (15 MFMAs), h2 plane i-3
-//            being complete afterwards (21 groups in 24 slots, 3 per wave);
+//            being complete afterwards (21 groups);
 //   layer 3: one banded block per wave (2 output rows, see NSTEP3B): six MFMAs on h2 plane i-4,
 //            whose three plane-offset partials move one 16-lane group per step (a shuffle), so
 //            output plane i-5 is complete;
 //   input:   stores plane i+1 (rounded to BF16, loaded three steps earlier) into a four-plane
 //            ring and requests plane i+4.
 // The layers of one step touch different planes -- h1 and h2 are double-buffered in LDS -- so a
-// step needs a single barrier.  A step of the march's steady state (every layer active: all
-// steps but the first seven and the last two) is one basic block, so the compiler can overlap a
-// wave's three layers: it has no plane-range tests, the spare layer-2 slots compute a clamped
-// group, and lanes with nothing to store write to a scratch slot instead of branching.  The h2
+// step needs a single barrier, and a step lasts as long as its slowest wave.  The groups are
+// dealt by measured cost (a layer-2 group costs about three layer-1 groups, a layer-3 block about
+// as much as a layer-2 group) rather than evenly:
+//   waves 0-4 ("heavy"): layer-2 groups w, w+8, w+16, layer-1 groups w, w+8, layer-3 block w;
+//   waves 5-7 ("light"): layer-2 groups w, w+8, layer-1 groups w, w+8 and three of 16..24,
+//                        layer-3 block w.
+// Dealing 3 layer-2 groups per wave (three spare slots) and the 25th layer-1 group to wave 0
+// took 4-5 % longer.  Each role runs its own copy of the march (a wave-uniform branch before
+// it), so that a step of the march's steady state (every layer active: all steps but the first
+// seven and the last two) is one basic block and the compiler can overlap a wave's three layers:
+// it has no plane-range tests, and lanes with nothing to store write to a scratch slot instead of
+// branching.  The h2
 // partial sums of unfinished planes stay in registers, rotating through three slots (the march
 // is unrolled three times so that the slots are compile-time registers).  h1 is computed on a
 // 20 x 20 and h2 on an 18 x 18 footprint (the halos the next layer needs).  Every LDS address
@@ -441,10 +449,20 @@ constexpr int F0X = FX + 6, F0Y = FY + 6;   // input footprint (484 voxels)
 constexpr int F2V = F2X * F2Y, F1V = F1X * F1Y, F0V = F0X * F0Y;
 constexpr int F2G = (F2V + 15) / 16, F1G = F1V / 16, F3G = FX * FY / 16;
 constexpr int FWAVES = 8, FTHREADS = 64 * FWAVES;  // waves per fused workgroup (2 per SIMD)
-constexpr int G1W = (F1G + FWAVES - 1) / FWAVES;    // voxel groups per wave: layer 1 (at most 4),
-constexpr int G2W = (F2G + FWAVES - 1) / FWAVES;    //   layer 2 (3),
-constexpr int F3B = FY / L3ROWS;                   //   layer 3: row blocks (one per wave)
+constexpr int G2W = (F2G + FWAVES - 1) / FWAVES;    // layer-2 groups of a heavy wave (3)
+constexpr int FHEAVY = F2G - (G2W - 1) * FWAVES;    // heavy waves (5): the rest have G2W - 1 groups
+constexpr int FLIGHT = FWAVES - FHEAVY;             // light waves (3)
+constexpr int G1H = 2, G1X = 3;                     // layer-1 groups: heavy, extra per light wave
+constexpr int G1W = G1H + G1X;                      // layer-1 slots of a light wave (5)
+static_assert(F2G == 21 && F1G == G1H * FWAVES + G1X * FLIGHT, "group deal of the 16 x 16 footprint");
+constexpr int F3B = FY / L3ROWS;                   // layer 3: row blocks (one per wave)
 static_assert(F3B == FWAVES, "one layer-3 row block per wave");
+// layer-1 group of slot k of wave w (k < G1H for heavy waves): w + 8k, then the light wave's
+// share of groups 16..24 dealt round robin
+__device__ __forceinline__ int fused_g1(int w, int k)
+{
+    return k < G1H ? w + FWAVES * k : G1H * FWAVES + (w - FHEAVY) + FLIGHT * (k - G1H);
+}
 constexpr int TPL = F0Y * F1X;                     // x-triplets per input-window plane (440)
 constexpr int VS = 16;  // h1 / h2 voxel record in LDS: 16 BF16 channels (32 bytes)
 constexpr int H1S = F1V * VS + 8, H2S = F2V * VS + 8;  // plane buffers + 8 zero channels
@@ -494,20 +512,22 @@ __global__ __launch_bounds__(FTHREADS) __attribute__((amdgpu_waves_per_eu(2, 2))
     }
 
     // ---- plane-independent per-lane facts ----
-    // layer 1: group k is g = w + 8k, voxel v = 16 g + n of the 20 x 20 footprint (T row = v);
-    // tap lines 2c, 2c+1 and 8 of this lane's operand row, window planes lX / 3
+    // layer 1: slot k holds group g = fused_g1(w, k), voxel v = 16 g + n of the 20 x 20 footprint
+    // (T row = v); tap lines 2c, 2c+1 and 8 of this lane's operand row, window planes lX / 3
+    const bool heavy = w < FHEAVY;
     const int v1 = 16 * w + n;
     const int lA = 2 * c, lB = 2 * c + 1;
     const int rA = (lA % 3) * F1X + v1, rB = (lB % 3) * F1X + v1, rC = 2 * F1X + v1;
     const int pA = lA / 3, pB = lB / 3;
-    unsigned ok1 = 0;  // bit k: group k exists and its voxel lies inside the grid (x, y)
+    unsigned ok1 = 0;  // bit k: slot k holds a group whose voxel lies inside the grid (x, y)
 #pragma unroll
     for (int k = 0; k < G1W; ++k) {
-        const int g = w + FWAVES * k, v = v1 + 16 * FWAVES * k;
+        const int g = k < G1H || !heavy ? fused_g1(w, k) : F1G, v = 16 * g + n;
         const int gx = ox - 2 + v % F1X, gy = oy - 2 + v / F1X;
         if (g < F1G && gx >= 0 && gx < nx && gy >= 0 && gy < ny) ok1 |= 1u << k;
     }
-    // layer 2: group k is g = w + 8k; operand base (hi offset included) and validity
+    // layer 2: group k is g = w + 8k (k < 2 for light waves); operand base (hi offset included)
+    // and validity
     int b2[G2W];
     unsigned ok2 = 0;
 #pragma unroll
@@ -580,8 +600,9 @@ __global__ __launch_bounds__(FTHREADS) __attribute__((amdgpu_waves_per_eu(2, 2))
     F3 pre0 = fetch(z0 - 3), pre1 = fetch(z0 - 2), pre2 = fetch(z0 - 1);
 
     // step i; rotating slot R holds the partial sums the step starts (h2 plane i-1, output plane
-    // i-3), R+2 those it continues, R+1 those it completes
-    auto step = [&](auto R, int i, auto steadyc) __attribute__((always_inline)) {
+    // i-3), R+2 those it continues, R+1 those it completes.  HEAVY: the wave's role (see above).
+    auto step = [&](auto R, int i, auto steadyc, auto heavyc) __attribute__((always_inline)) {
+        constexpr bool HEAVY = decltype(heavyc)::value;
         constexpr int NXT = decltype(R)::value, CUR = (NXT + 2) % 3, PRV = (NXT + 1) % 3;
         constexpr bool STEADY = decltype(steadyc)::value;  // every layer active: no range tests
         *(trip ? &T[(i + 1) & 3][threadIdx.x] : &Trash[threadIdx.x]) = to_triplet(i + 1, pre0);
@@ -598,7 +619,7 @@ __global__ __launch_bounds__(FTHREADS) __attribute__((amdgpu_waves_per_eu(2, 2))
             uint16_t *Hw = H2[pd & 1] + v1 * VS + 4 * c;
 #pragma unroll
             for (int k = 0; k < G2W; ++k) {
-                {  // groups past the footprint compute a clamped group and store nothing
+                if (HEAVY || k < G2W - 1) {
                     u32x4 raw[NSTEP3F];
                     load_pairs(Hq, std::integral_constant<int, F1X>{}, F1V * VS, b2[k], raw);
                     f32x4 an = {0.f, 0.f, 0.f, 0.f};
@@ -627,8 +648,8 @@ __global__ __launch_bounds__(FTHREADS) __attribute__((amdgpu_waves_per_eu(2, 2))
             uint16_t *Hw = H1[q1 & 1] + v1 * VS + 4 * c;
 #pragma unroll
             for (int k = 0; k < G1W; ++k) {
-                if (FWAVES * (k + 1) <= F1G || w + FWAVES * k < F1G) {
-                    const int o16 = 16 * FWAVES * k;
+                if (!HEAVY || k < G1H) {
+                    const int o16 = 16 * (fused_g1(w, k) - w);
                     const u32x2 ta = TA[o16], tb = TB[o16], tc = TC[o16];
                     u32x4 raw;
                     raw[0] = ta[0];
@@ -670,27 +691,31 @@ __global__ __launch_bounds__(FTHREADS) __attribute__((amdgpu_waves_per_eu(2, 2))
         }
         __syncthreads();
     };
-    // steps z0-4 .. z1+4, unrolled by three for the rotating slots
     // steps z0-4 .. z1+4; the rotating slot of step i is (i - z0 + 4) % 3.  Steps z0+3 .. z1+2
     // run every layer (the steady state, no range tests); the first 7 and last 2 test ranges.
+    // Every wave takes the same steps and barriers; heavy and light waves run their own copy.
     using Tr = std::true_type;
     using Fa = std::false_type;
-    auto any_step = [&](int i, auto steadyc) __attribute__((always_inline)) {
-        switch ((i - z0 + 4) % 3) {
-        case 0: step(Slot<0>{}, i, steadyc); break;
-        case 1: step(Slot<1>{}, i, steadyc); break;
-        default: step(Slot<2>{}, i, steadyc); break;
+    auto march = [&](auto heavyc) __attribute__((always_inline)) {
+        auto any_step = [&](int i, auto steadyc) __attribute__((always_inline)) {
+            switch ((i - z0 + 4) % 3) {
+            case 0: step(Slot<0>{}, i, steadyc, heavyc); break;
+            case 1: step(Slot<1>{}, i, steadyc, heavyc); break;
+            default: step(Slot<2>{}, i, steadyc, heavyc); break;
+            }
+        };
+        int i = z0 - 4;
+        for (; i <= z0 + 2; ++i) any_step(i, Fa{});
+        for (; i + 2 <= z1 + 2; i += 3) {  // slots 1, 2, 0 (z0+3 is slot 1)
+            step(Slot<1>{}, i, Tr{}, heavyc);
+            step(Slot<2>{}, i + 1, Tr{}, heavyc);
+            step(Slot<0>{}, i + 2, Tr{}, heavyc);
         }
+        for (; i <= z1 + 2; ++i) any_step(i, Tr{});
+        for (; i <= z1 + 4; ++i) any_step(i, Fa{});
     };
-    int i = z0 - 4;
-    for (; i <= z0 + 2; ++i) any_step(i, Fa{});
-    for (; i + 2 <= z1 + 2; i += 3) {  // slots 1, 2, 0 (z0+3 is slot 1)
-        step(Slot<1>{}, i, Tr{});
-        step(Slot<2>{}, i + 1, Tr{});
-        step(Slot<0>{}, i + 2, Tr{});
-    }
-    for (; i <= z1 + 2; ++i) any_step(i, Tr{});
-    for (; i <= z1 + 4; ++i) any_step(i, Fa{});
+    if (heavy) march(Tr{});
+    else march(Fa{});
 }
 
 }  // namespace cnn
