@@ -459,10 +459,34 @@ constexpr int F3B = FY / L3ROWS;                   // layer 3: row blocks (one p
 static_assert(F3B == FWAVES, "one layer-3 row block per wave");
 // layer-1 group of slot k of wave w (k < G1H for heavy waves): w + 8k, then the light wave's
 // share of groups 16..24 dealt round robin
-__device__ __forceinline__ int fused_g1(int w, int k)
+__host__ __device__ constexpr int fused_g1(int w, int k)
 {
     return k < G1H ? w + FWAVES * k : G1H * FWAVES + (w - FHEAVY) + FLIGHT * (k - G1H);
 }
+// the deal covers every layer-1 group (slots 0..G1H-1 of every wave, G1H..G1W-1 of the light
+// waves) and every layer-2 group (w + 8k, k < G2W for heavy waves, k < G2W - 1 for light) once
+constexpr bool fused_deal_exact()
+{
+    int n1[F1G] = {}, n2[F2G] = {};
+    for (int w = 0; w < FWAVES; ++w) {
+        for (int k = 0; k < (w < FHEAVY ? G1H : G1W); ++k) {
+            const int g = fused_g1(w, k);
+            if (g < 0 || g >= F1G) return false;
+            ++n1[g];
+        }
+        for (int k = 0; k < (w < FHEAVY ? G2W : G2W - 1); ++k) {
+            const int g = w + FWAVES * k;
+            if (g >= F2G) return false;
+            ++n2[g];
+        }
+    }
+    for (int g = 0; g < F1G; ++g)
+        if (n1[g] != 1) return false;
+    for (int g = 0; g < F2G; ++g)
+        if (n2[g] != 1) return false;
+    return true;
+}
+static_assert(fused_deal_exact(), "every layer-1 and layer-2 group is dealt to exactly one wave slot");
 constexpr int TPL = F0Y * F1X;                     // x-triplets per input-window plane (440)
 constexpr int VS = 16;  // h1 / h2 voxel record in LDS: 16 BF16 channels (32 bytes)
 constexpr int H1S = F1V * VS + 8, H2S = F2V * VS + 8;  // plane buffers + 8 zero channels
