@@ -323,6 +323,25 @@ def measure_fixed_iterations(ctx, n1=128, iters=150, steps=3, reps=20):
     return out
 
 
+def measure_spmv_fp16(ctx, n, nnz, reps):
+    """Roofline of the mixed and flexible solvers' SpMV (km_spmv; kf_spmv runs the same row loop):
+    FP16 values, one-byte column codes, FP32 x and y, FP64 row sums and the p.Ap partials, timed
+    with HIP events over `reps` back-to-back launches on the context's stream.  Format bytes:
+    2 + 1 per entry, the row pointers, x once and y (FP32)."""
+    import numpy as np
+    import hpccg_amd
+    x = hpccg_amd.DeviceArray.from_numpy(ctx, np.random.default_rng(2).standard_normal(n).astype(np.float32))
+    y = hpccg_amd.DeviceArray(ctx, n, np.float32)
+    ms = ctx.time_spmv_mixed(x, y, reps=reps)
+    fmt = ctx.matrix_format()
+    Bf = spmv_bytes(n, nnz, s_val=2, s_idx=1 if fmt["ncodes"] else 4, s_x=4, s_y=4)
+    gbs = Bf / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "km_spmv (FP16 values, one-byte column codes, FP32 x/y, FP64 row sums, p.Ap partials)",
+            "format_bytes_per_launch": Bf, "avg_ms": round(ms, 5), "launches": reps}
+
+
 def measure_network(ctx, grid, reps):
     """Roofline of the flexible CG's dominant kernel, the fused network inference z = net(u)
     (cnn::k_fused_fwd), on a random grid^3 input: `reps` back-to-back applications timed with
@@ -498,6 +517,9 @@ def main():
 
     roofline = measure_roofline(ctx, xs, n, nnz, n1, args.spmv_reps, b_h=b_h, tol=REL_TOL * normb,
                                 live_traffic=live_traffic)
+    roofline_fp16 = measure_spmv_fp16(ctx, n, nnz, args.spmv_reps)
+    log(f"fp16 spmv: {roofline_fp16['avg_ms']} ms, {roofline_fp16['achieved']} GB/s of format bytes "
+        f"({100 * roofline_fp16['frac']:.1f}% of peak)")
 
     # batched solves (64 systems, extents ~ U{16..96}, x* ~ U(-1,1), to 1e-8 each; the solve
     # enqueues graph chunks sized from the batch's residual rate, and a repeated solve the count
@@ -536,14 +558,14 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(grid, b_h, results["fp64_cg"]["niters"], log_dir)
         log(f"cpu baseline: {cpu}")
-    # 128^3, 150 iterations (the reference's fixed-iteration benchmark at that size); replaces
-    # the context's matrix, so it runs last
     # the flexible CG's dominant kernel: the network (its weights are set above)
     roofline_net = None
     if weights is not None:
         roofline_net = measure_network(ctx, grid, args.spmv_reps)
         log(f"network: {roofline_net['avg_ms']} ms per application, {roofline_net['achieved']} TFLOP/s "
             f"({100 * roofline_net['frac']:.1f}% of BF16 peak)")
+    # 128^3, 150 iterations (the reference's fixed-iteration benchmark at that size); replaces
+    # the context's matrix, so it runs last
     fixed150 = measure_fixed_iterations(ctx, 128, 150, steps=max(1, args.steps))
 
     out = {
@@ -563,6 +585,7 @@ def main():
                    "solver": headline, "rows": n, "nnz": nnz},
         "roofline": roofline,
         "roofline_network": roofline_net,
+        "roofline_spmv_fp16": roofline_fp16,
         "cpu_baseline": cpu,
         "setup": setup,
         "solvers": results,

@@ -28,6 +28,15 @@ def test_spmv_bytes_closed_forms(bench, n1, csr, fmt):
     assert csr - fmt == 3 * nnz
 
 
+@pytest.mark.parametrize("n1,fmt16", [(128, 192_394_732), (256, 1_549_691_884)])
+def test_fp16_spmv_format_bytes(bench, n1, fmt16):
+    """The mixed SpMV's format: 2-byte values and 1-byte codes (3 per entry), the row pointers,
+    FP32 x once and FP32 y."""
+    n, nnz = n1 ** 3, (3 * n1 - 2) ** 3
+    assert bench.spmv_bytes(n, nnz, s_val=2, s_idx=1, s_x=4, s_y=4) == fmt16
+    assert fmt16 == 3 * nnz + 4 * (n + 1) + 8 * n
+
+
 def _write_counter_csv(path, rows):
     cols = ["Dispatch_Id", "Kernel_Name", "Counter_Name", "Counter_Value"]
     with open(path, "w", newline="") as f:
