@@ -332,6 +332,9 @@ def measure_spmv_fp16(ctx, n, nnz, reps):
     import hpccg_amd
     x = hpccg_amd.DeviceArray.from_numpy(ctx, np.random.default_rng(2).standard_normal(n).astype(np.float32))
     y = hpccg_amd.DeviceArray(ctx, n, np.float32)
+    # the rate rises over the first ~200 launches on fresh buffers (0.298 -> 0.274 ms at 256^3
+    # in twelve rounds of 20), so ten untimed rounds come first
+    warm = [ctx.time_spmv_mixed(x, y, reps=reps) for _ in range(10)]
     ms = ctx.time_spmv_mixed(x, y, reps=reps)
     fmt = ctx.matrix_format()
     Bf = spmv_bytes(n, nnz, s_val=2, s_idx=1 if fmt["ncodes"] else 4, s_x=4, s_y=4)
@@ -339,7 +342,8 @@ def measure_spmv_fp16(ctx, n, nnz, reps):
     return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
             "kernel": "km_spmv (FP16 values, one-byte column codes, FP32 x/y, FP64 row sums, p.Ap partials)",
-            "format_bytes_per_launch": Bf, "avg_ms": round(ms, 5), "launches": reps}
+            "format_bytes_per_launch": Bf, "avg_ms": round(ms, 5), "launches": reps,
+            "warmup_ms": [round(w, 4) for w in warm]}
 
 
 def measure_network(ctx, grid, reps):
@@ -355,6 +359,11 @@ def measure_network(ctx, grid, reps):
     z = hpccg_amd.DeviceArray(ctx, nvox, np.float32)
     for _ in range(10):
         ctx.cnn_apply_dev(grid, u, 1.0, z)
+    # The kernel is compute-bound and its rate follows the GPU's clock, which rises under
+    # sustained load: back to back it takes ~0.46, 0.43, 0.40 ms in the first rounds of 20 at
+    # 256^3 and settles at ~0.387 ms by the fourth.  Eight untimed rounds come first; the
+    # timed round is the steady state, and the rounds before it are reported.
+    warm = [ctx.time_cnn_apply(grid, u, z, reps=reps) for _ in range(8)]
     ms = ctx.time_cnn_apply(grid, u, z, reps=reps)
     flop = 2.0 * 27 * (16 + 16 * 16 + 16) * nvox
     tflops = flop / (ms * 1e-3) / 1e12
@@ -362,7 +371,9 @@ def measure_network(ctx, grid, reps):
             "frac": round(tflops / 2500.0, 4), "traffic": None, "kernel": "cnn::k_fused_fwd (3-layer BF16 CNN, "
             "MFMA 16x16x32, FP32 accumulation)", "grid": list(grid), "flop_per_launch": flop,
             "hbm_bytes_per_launch": 8 * nvox, "avg_ms": round(ms, 5), "ms_per_application": round(ms, 4),
-            "launches": reps}
+            "launches": reps, "warmup_ms": [round(w, 4) for w in warm],
+            "note": "steady state under back-to-back load; inside the flexible-CG solve, where it alternates "
+                    "with memory-bound kernels, the kernel trace shows 0.44-0.45 ms per application"}
 
 
 def main():

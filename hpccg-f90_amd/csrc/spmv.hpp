@@ -54,7 +54,10 @@ constexpr int kTrueThreads = kSlice * kTrueWaves;
 constexpr int kTrueLoop = 4;
 constexpr int kStencilK = 27;         // slice width of the 27-point stencil: fully unrolled path
 constexpr int kGroupF64 = 2;          // FP64 copy: 2 entries per lane per load (16-byte value loads)
-constexpr int kGroupF16 = 1;          // FP16 copy: 1 entry per lane per load (measured best)
+// FP16 copy: 1 entry per lane per load.  Groups of 2, 4 and 8 entries per lane (4-, 8- and
+// 16-byte loads) measured the same at 256^3: mixed SpMV 0.283-0.288 ms for every group size,
+// the same solve times, bitwise the same results.
+constexpr int kGroupF16 = 1;
 // FP16-copy SpMVs of the CG loops (km_spmv, kf_spmv) are compiled for 7 waves per SIMD (at most
 // 72 VGPRs).  Measured at 256^3: 0.293 ms with the compiler's own choice (80 VGPRs, 6 waves),
 // 0.279 ms with 7, 0.316 ms with 5 and 0.421 ms with 8.  The FP64 SpMV is fastest at its own 5.
