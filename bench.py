@@ -393,7 +393,7 @@ def main():
                     help="only build the matrix and time the SpMV and the copy (for counter runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
+        print("bench.py measures one GPU only: the library has no multi-GPU path "
               "(a multi-GPU run would shard the grid into z-slabs, one per GPU); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
