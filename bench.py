@@ -189,7 +189,8 @@ def counter_kib(csv_dir, counter):
                 if row.get("Counter_Name") != counter:
                     continue
                 name = row.get("Kernel_Name", "")
-                key = "spmv" if "k_cg_spmv<true>" in name else ("copy" if "k_copy16" in name else None)
+                key = ("spmv" if "k_cg_spmv<true>" in name else "spmv16" if "km_spmv(" in name
+                       else "copy" if "k_copy16" in name else None)
                 if key:
                     vals.setdefault(key, []).append(float(row["Counter_Value"]))
     return {k: (sum(v) / len(v), len(v)) for k, v in vals.items()}
@@ -199,8 +200,9 @@ def measure_traffic(n1, timeout=180):
     """HBM bytes of one k_cg_spmv<true> launch at n1^3, measured now: FETCH_SIZE and WRITE_SIZE
     each from a rocprofv3 --pmc run of its own (a child process running `bench.py --spmv-only`,
     started before this process uses the GPU), each scaled by what the same counter reads for the
-    copy kernel of known size that the same run times.  None when the profiler is absent or a
-    run fails (the caller then reports the stored measurement)."""
+    copy kernel of known size that the same run times.  The same runs give the FP16 SpMV's
+    (km_spmv) bytes as `fp16_*`.  None when the profiler is absent or a run fails (the caller then
+    reports the stored measurement)."""
     exe = shutil.which("rocprofv3")
     if not exe or os.environ.get("HPCCG_BENCH_NO_COUNTERS"):
         return None
@@ -232,9 +234,14 @@ def measure_traffic(n1, timeout=180):
         out[f"copy_{key}_kib"] = round(kib["copy"][0], 1)
         out[f"{key}_scale"] = round(scale, 6)
         out[f"{key}_bytes"] = int(round(kib["spmv"][0] * 1024.0 * scale))
+        if "spmv16" in kib:
+            out[f"fp16_{key}_kib_per_dispatch"] = round(kib["spmv16"][0], 1)
+            out[f"fp16_{key}_bytes"] = int(round(kib["spmv16"][0] * 1024.0 * scale))
         out[f"{key}_command"] = " ".join(["rocprofv3", "--pmc", ctr, "--", "python3", "bench.py", "--spmv-only",
                                           "--spmv-reps", "5", "--grid", str(n1)])
     out["traffic_bytes"] = out["fetch_bytes"] + out["write_bytes"]
+    if "fp16_fetch_bytes" in out and "fp16_write_bytes" in out:
+        out["fp16_traffic_bytes"] = out["fp16_fetch_bytes"] + out["fp16_write_bytes"]
     return out
 
 
@@ -462,7 +469,8 @@ def main():
         del x32, y32
     log(f"setup {setup}: grid {grid}, n={n}, nnz={nnz}, ||b||={normb:.6e}")
     if args.spmv_only:
-        print(json.dumps({"roofline": measure_roofline(ctx, xs, n, nnz, n1, args.spmv_reps)}), flush=True)
+        print(json.dumps({"roofline": measure_roofline(ctx, xs, n, nnz, n1, args.spmv_reps),
+                          "roofline_spmv_fp16": measure_spmv_fp16(ctx, n, nnz, args.spmv_reps)}), flush=True)
         ctx.close()
         return
 
@@ -529,6 +537,10 @@ def main():
     roofline = measure_roofline(ctx, xs, n, nnz, n1, args.spmv_reps, b_h=b_h, tol=REL_TOL * normb,
                                 live_traffic=live_traffic)
     roofline_fp16 = measure_spmv_fp16(ctx, n, nnz, args.spmv_reps)
+    if live_traffic and "fp16_traffic_bytes" in live_traffic:
+        roofline_fp16["traffic"] = live_traffic["fp16_traffic_bytes"]
+        roofline_fp16["traffic_source"] = {k: v for k, v in live_traffic.items() if k.startswith("fp16_")
+                                           or k.endswith("_scale")}
     log(f"fp16 spmv: {roofline_fp16['avg_ms']} ms, {roofline_fp16['achieved']} GB/s of format bytes "
         f"({100 * roofline_fp16['frac']:.1f}% of peak)")
 

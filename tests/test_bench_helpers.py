@@ -57,9 +57,12 @@ def test_counter_kib_means_per_kernel(bench, tmp_path):
         (4, "void hpccg::k_cg_spmv<false>(hpccg::SellRef<double>)", "FETCH_SIZE", "9999.0"),
         (5, "hpccg::k_generate(int, int, int)", "FETCH_SIZE", "7.0"),
         (6, "hpccg::k_copy16(unsigned int __vector(4) const*)", "WRITE_SIZE", "80.0"),
+        (7, "hpccg::km_spmv(hpccg::SellRef<__half>, float const*, float*, hpccg::MixState const*, double*)",
+         "FETCH_SIZE", "40.0"),
+        (8, "hpccg::km_spmv_finish(double const*, int, hpccg::MixState*)", "FETCH_SIZE", "1.0"),
     ])
     got = bench.counter_kib(str(tmp_path), "FETCH_SIZE")
-    assert got == {"spmv": (200.0, 2), "copy": (50.0, 1)}
+    assert got == {"spmv": (200.0, 2), "copy": (50.0, 1), "spmv16": (40.0, 1)}
     assert bench.counter_kib(str(tmp_path), "WRITE_SIZE") == {"copy": (80.0, 1)}
 
 
