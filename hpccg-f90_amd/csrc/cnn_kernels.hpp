@@ -425,8 +425,8 @@ __global__ __launch_bounds__(256) void k_wgrad(const uint16_t *__restrict__ dout
 //            ring and requests plane i+4.
 // The layers of one step touch different planes -- h1 and h2 are double-buffered in LDS -- so a
 // step needs a single barrier, and a step lasts as long as its slowest wave.  The groups are
-// dealt by measured cost (a layer-2 group costs about three layer-1 groups, a layer-3 block about
-// as much as a layer-2 group) rather than evenly:
+// dealt by measured cost (a layer-1 group costs well under a layer-2 group, a layer-3 block
+// about as much as a layer-2 group) rather than evenly:
 //   waves 0-4 ("heavy"): layer-2 groups w, w+8, w+16, layer-1 groups w, w+8, layer-3 block w;
 //   waves 5-7 ("light"): layer-2 groups w, w+8, layer-1 groups w, w+8 and three of 16..24,
 //                        layer-3 block w.
@@ -435,9 +435,8 @@ __global__ __launch_bounds__(256) void k_wgrad(const uint16_t *__restrict__ dout
 // it), so that a step of the march's steady state (every layer active: all steps but the first
 // seven and the last two) is one basic block and the compiler can overlap a wave's three layers:
 // it has no plane-range tests, and lanes with nothing to store write to a scratch slot instead of
-// branching.  The h2
-// partial sums of unfinished planes stay in registers, rotating through three slots (the march
-// is unrolled three times so that the slots are compile-time registers).  h1 is computed on a
+// branching.  The h2 partial sums of unfinished planes stay in registers, rotating through three
+// slots (the march is unrolled three times so that the slots are compile-time registers).  h1 is computed on a
 // 20 x 20 and h2 on an 18 x 18 footprint (the halos the next layer needs).  Every LDS address
 // and grid-boundary test that does not depend on the plane is formed once, before the march.
 // Each accumulator receives its K-steps in the order of the layer-wise k_conv, so the output is
