@@ -180,6 +180,27 @@ def test_fcg_learned_gain_and_accuracy(ctx):
     assert res["fcg_iters"] <= 0.5 * nit_plain, (res["fcg_iters"], nit_plain)
 
 
+@pytest.mark.parametrize("dims", [(36, 20, 28), (17, 45, 9)])
+def test_fcg_learned_on_ragged_grids(ctx, dims):
+    """Grids whose extents are not multiples of the network's 16 x 16 footprint (partial
+    columns in x and y, z-chunks of uneven length): the flexible CG meets 1e-10 in the FP64
+    true residual (checked by the oracle) without falling back, in fewer iterations than plain
+    CG."""
+    from hpccg_amd import learned
+    ctx.cnn_set_weights(learned.load())
+    A = oracle.Problem(*dims)
+    b = oracle.spmv(A, oracle.xstar_sines(*dims, 5))
+    normb = math.sqrt(oracle.ddot1(b))
+    ctx.upload(A.rowindex, A.colindex, A.values)
+    bd = hpccg_amd.DeviceArray.from_numpy(ctx, b)
+    xd = hpccg_amd.DeviceArray.from_numpy(ctx, np.zeros(A.n))
+    res = ctx.fcg_learned_dev(dims, bd, xd, max_iter=500, tolerance=1e-10 * normb)
+    rel = math.sqrt(oracle.ddot1(b - oracle.spmv(A, xd.numpy()))) / normb
+    assert rel <= 1e-10, (rel, res)
+    _, nit_plain, _, _, _ = oracle.cg(A, b, max_iter=500, tolerance=1e-10 * normb)
+    assert not res["fell_back"] and res["fcg_iters"] < nit_plain, (res, nit_plain)
+
+
 def test_training_pipeline_selects_a_network_that_beats_plain_cg(ctx):
     """hpccg_amd.learned.train end to end at a reduced size (48^3 pairs, 4 initialisations):
     the selected network needs fewer flexible-CG iterations than plain CG on the validation
