@@ -487,6 +487,7 @@ constexpr bool fused_deal_exact()
 }
 static_assert(fused_deal_exact(), "every layer-1 and layer-2 group is dealt to exactly one wave slot");
 constexpr int TPL = F0Y * F1X;                     // x-triplets per input-window plane (440)
+static_assert(64 + TPL <= FTHREADS, "the input loader (one triplet per thread) fits waves 1..7");
 constexpr int VS = 16;  // h1 / h2 voxel record in LDS: 16 BF16 channels (32 bytes)
 constexpr int H1S = F1V * VS + 8, H2S = F2V * VS + 8;  // plane buffers + 8 zero channels
 
@@ -570,12 +571,16 @@ __global__ __launch_bounds__(FTHREADS) __attribute__((amdgpu_waves_per_eu(2, 2))
     for (int j = 0; j < L3ROWS; ++j)
         if (ox + n < nx && oy + L3ROWS * bk + j < ny) ok3 |= 1u << j;
 
-    // input loader: thread t < TPL owns triplet t (window row t / F1X, columns t % F1X .. +2).
-    // The three loads read clamped, always valid addresses without branches, so they stay in
-    // flight until the plane is stored three steps later; the zero padding outside the grid and
-    // the BF16 rounding are applied then.
-    const bool trip = threadIdx.x < TPL;
-    const int tgy = oy - 3 + (int)threadIdx.x / F1X, tgx0 = ox - 3 + (int)threadIdx.x % F1X;
+    // input loader: thread 64 + t, t < TPL, owns triplet t (window row t / F1X, columns t % F1X
+    // .. +2) -- waves 1-7, so that wave 0, a heavy wave sharing its SIMD with heavy wave 4, loads
+    // nothing (measured 0.381-0.382 against 0.385-0.386 ms with the loader on waves 0-6).  The
+    // three loads read clamped, always valid addresses without branches, so they stay in flight
+    // until the plane is stored three steps later; the zero padding outside the grid and the BF16
+    // rounding are applied then.
+    const int tl = (int)threadIdx.x - 64;
+    const bool trip = tl >= 0 && tl < TPL;
+    const int tlc = trip ? tl : 0;
+    const int tgy = oy - 3 + tlc / F1X, tgx0 = ox - 3 + tlc % F1X;
     const bool trow = trip && tgy >= 0 && tgy < ny;
     const bool tx0 = tgx0 >= 0 && tgx0 < nx, tx1 = tgx0 + 1 >= 0 && tgx0 + 1 < nx, tx2 = tgx0 + 2 >= 0 && tgx0 + 2 < nx;
     const float *trowp = in + (int64_t)min(max(tgy, 0), ny - 1) * nx;
@@ -628,7 +633,7 @@ __global__ __launch_bounds__(FTHREADS) __attribute__((amdgpu_waves_per_eu(2, 2))
         constexpr bool HEAVY = decltype(heavyc)::value;
         constexpr int NXT = decltype(R)::value, CUR = (NXT + 2) % 3, PRV = (NXT + 1) % 3;
         constexpr bool STEADY = decltype(steadyc)::value;  // every layer active: no range tests
-        *(trip ? &T[(i + 1) & 3][threadIdx.x] : &Trash[threadIdx.x]) = to_triplet(i + 1, pre0);
+        *(trip ? &T[(i + 1) & 3][tlc] : &Trash[threadIdx.x]) = to_triplet(i + 1, pre0);
         pre0 = pre1;
         pre1 = pre2;
         pre2 = fetch(i + 4);
