@@ -45,6 +45,14 @@ struct BatchState {
     int replace_every, max_iter;
 };
 
+// Where a slice lies, in one 16-byte record (one scalar load per slice instead of a chain of
+// system-table loads): its first global row, its system and how many of its 64 rows exist.
+struct SliceInfo {
+    int64_t row0;
+    int32_t sys;
+    int32_t nvalid;
+};
+
 struct Batch {
     int S = 0;
     std::vector<int32_t> dims;          // 3*S
@@ -53,14 +61,13 @@ struct Batch {
     int32_t *rowindex = nullptr;        // E+1, global 1-based
     int32_t *colindex = nullptr;        // nnz, global 1-based
     __half *values = nullptr;           // nnz (27 / -1 exactly)
-    int64_t *row_off = nullptr;         // device copy of row_off_h
     // sliced-ELL copy (spmv.hpp), slices never straddle two systems
     int64_t *slice_ptr = nullptr;       // [total slices + 1] slot offsets
     uint8_t *code = nullptr;            // column codes (spmv.hpp), offsets relative to each system
     int32_t *tabs = nullptr;            // [S][kBatchTab] each system's sorted offset table
     __half *sval = nullptr;
     int *slice_off = nullptr;           // [S+1] first slice of each system
-    int *slice_sys = nullptr;           // [NS] system of each slice
+    SliceInfo *sinfo = nullptr;         // [NS] each slice's system, first row and valid rows
     int NS = 0;                         // slices of the whole batch
     int64_t slots = 0;
     float *r = nullptr, *p = nullptr, *Ap = nullptr;
@@ -131,14 +138,13 @@ struct BatchSell {
     const __half *val;
     const int32_t *rowptr;  // global 1-based row pointer
     const int *slice_off;   // first slice of each system
-    const int *slice_sys;   // system of each slice
-    const int64_t *row_off; // first row of each system
+    const SliceInfo *sinfo; // [NS] system, first row and valid rows of each slice
     int NS;
 };
 
 static inline BatchSell batch_sell(const Batch *B)
 {
-    return BatchSell{B->slice_ptr, B->code, B->tabs, B->sval, B->rowindex, B->slice_off, B->slice_sys, B->row_off, B->NS};
+    return BatchSell{B->slice_ptr, B->code, B->tabs, B->sval, B->rowindex, B->slice_off, B->sinfo, B->NS};
 }
 
 // Where slice gs lies: its system, the lane's global row and whether that row exists.
@@ -149,12 +155,11 @@ struct SlicePos {
 };
 __device__ __forceinline__ SlicePos slice_pos(const BatchSell &A, int gs, int lane)
 {
+    const SliceInfo si = A.sinfo[gs];
     SlicePos q;
-    q.s = A.slice_sys[gs];
-    const int64_t off = A.row_off[q.s], n = A.row_off[q.s + 1] - off;
-    const int64_t rl = (int64_t)(gs - A.slice_off[q.s]) * kSlice + lane;
-    q.valid = rl < n;
-    q.gr = off + rl;
+    q.s = si.sys;
+    q.valid = lane < si.nvalid;
+    q.gr = si.row0 + lane;
     return q;
 }
 
@@ -399,7 +404,6 @@ void free_batched(hpccg_ctx *c)
     dfree(B->rowindex);
     dfree(B->colindex);
     dfree(B->values);
-    dfree(B->row_off);
     dfree(B->slice_ptr);
     dfree(B->code);
     dfree(B->tabs);
@@ -410,7 +414,7 @@ void free_batched(hpccg_ctx *c)
     dfree(B->Ap);
     dfree(B->st);
     dfree(B->partials);
-    dfree(B->slice_sys);
+    dfree(B->sinfo);
     dfree(B->nactive);
     delete B;
     c->ext[0] = nullptr;
@@ -561,7 +565,6 @@ int hpccg_batch_setup(hpccg_ctx *c, int32_t count, const int32_t *dims)
     if (dmalloc(&B->rowindex, (B->E + 1) * sizeof(int32_t)) != hipSuccess ||
         dmalloc(&B->colindex, B->nnz * sizeof(int32_t)) != hipSuccess ||
         dmalloc(&B->values, B->nnz * sizeof(__half)) != hipSuccess ||
-        dmalloc(&B->row_off, (count + 1) * sizeof(int64_t)) != hipSuccess ||
         dmalloc(&B->r, B->E * sizeof(float)) != hipSuccess || dmalloc(&B->p, B->E * sizeof(float)) != hipSuccess ||
         dmalloc(&B->Ap, B->E * sizeof(float)) != hipSuccess ||
         dmalloc(&B->st, count * sizeof(BatchState)) != hipSuccess) {
@@ -569,7 +572,6 @@ int hpccg_batch_setup(hpccg_ctx *c, int32_t count, const int32_t *dims)
         set_error("batched: device allocation failed");
         return HPCCG_ERR_NOMEM;
     }
-    HPCCG_HIP_TRY(hipMemcpy(B->row_off, B->row_off_h.data(), (count + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
     for (int s = 0; s < count; ++s) {
         const int64_t n = B->row_off_h[s + 1] - B->row_off_h[s];
         hipLaunchKernelGGL(kb_generate, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, dims[3 * s],
@@ -600,18 +602,21 @@ int hpccg_batch_setup(hpccg_ctx *c, int32_t count, const int32_t *dims)
         return HPCCG_ERR_NOMEM;
     }
     B->NS = soff[count];
-    std::vector<int> ssys(std::max(B->NS, 1), 0);
+    std::vector<SliceInfo> sinfo(std::max(B->NS, 1), SliceInfo{0, 0, 0});
     for (int s = 0; s < count; ++s)
-        for (int i = soff[s]; i < soff[s + 1]; ++i) ssys[i] = s;
-    if (dmalloc(&B->slice_sys, ssys.size() * sizeof(int)) != hipSuccess ||
-        dmalloc(&B->partials, 2 * ssys.size() * sizeof(double)) != hipSuccess ||
+        for (int i = soff[s]; i < soff[s + 1]; ++i) {
+            const int64_t r0 = (int64_t)(i - soff[s]) * kSlice, rows = B->row_off_h[s + 1] - B->row_off_h[s];
+            sinfo[i] = SliceInfo{B->row_off_h[s] + r0, s, (int32_t)std::min<int64_t>(kSlice, rows - r0)};
+        }
+    if (dmalloc(&B->sinfo, sinfo.size() * sizeof(SliceInfo)) != hipSuccess ||
+        dmalloc(&B->partials, 2 * sinfo.size() * sizeof(double)) != hipSuccess ||
         dmalloc(&B->nactive, 2 * sizeof(int)) != hipSuccess ||
         hipHostMalloc(&B->snap, 2 * (size_t)count * sizeof(BatchState), hipHostMallocDefault) != hipSuccess) {
         free_batched(c);
         set_error("batched: device allocation failed");
         return HPCCG_ERR_NOMEM;
     }
-    HPCCG_HIP_TRY(hipMemcpy(B->slice_sys, ssys.data(), ssys.size() * sizeof(int), hipMemcpyHostToDevice));
+    HPCCG_HIP_TRY(hipMemcpy(B->sinfo, sinfo.data(), sinfo.size() * sizeof(SliceInfo), hipMemcpyHostToDevice));
     HPCCG_HIP_TRY(hipMemcpy(B->slice_ptr, sp.data(), sp.size() * sizeof(int64_t), hipMemcpyHostToDevice));
     HPCCG_HIP_TRY(hipMemcpy(B->slice_off, soff.data(), (count + 1) * sizeof(int), hipMemcpyHostToDevice));
     // each system's table: the distinct stencil offsets dz*nx*ny + dy*nx + dx, sorted
