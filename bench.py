@@ -353,6 +353,28 @@ def measure_spmv_fp16(ctx, n, nnz, reps):
             "warmup_ms": [round(w, 4) for w in warm]}
 
 
+def measure_spmv_stencil(ctx, grid, xs, n, csr_ms, reps):
+    """The matrix-free form of generate_matrix's operator (k_stencil_spmv), the comparison point
+    for the CSR SpMV: same y bit for bit (checked here), 16 bytes per row (x once and y), timed
+    with HIP events after three untimed rounds."""
+    import numpy as np
+    import hpccg_amd
+    y1 = hpccg_amd.DeviceArray(ctx, n)
+    y2 = hpccg_amd.DeviceArray(ctx, n)
+    ctx.spmv_dev(xs, y1)
+    ctx.spmv_stencil_dev(grid, xs, y2)
+    same = bool(np.array_equal(y1.numpy().view(np.uint64), y2.numpy().view(np.uint64)))
+    warm = [ctx.time_spmv_stencil(grid, xs, y2, reps=reps) for _ in range(3)]
+    ms = ctx.time_spmv_stencil(grid, xs, y2, reps=reps)
+    B = 16 * n
+    gbs = B / (ms * 1e-3) / 1e9
+    return {"kernel": "k_stencil_spmv (matrix-free 27-point operator, FP64)", "bound": "hbm",
+            "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+            "bytes_per_launch": B, "avg_ms": round(ms, 5), "launches": reps,
+            "warmup_ms": [round(w, 4) for w in warm], "csr_spmv_ms": round(csr_ms, 5),
+            "speedup_vs_csr_spmv": round(csr_ms / ms, 2), "bitwise_equal_to_csr_spmv": same}
+
+
 def measure_network(ctx, grid, reps):
     """Roofline of the flexible CG's dominant kernel, the fused network inference z = net(u)
     (cnn::k_fused_fwd), on a random grid^3 input: `reps` back-to-back applications timed with
@@ -536,6 +558,9 @@ def main():
 
     roofline = measure_roofline(ctx, xs, n, nnz, n1, args.spmv_reps, b_h=b_h, tol=REL_TOL * normb,
                                 live_traffic=live_traffic)
+    spmv_stencil = measure_spmv_stencil(ctx, grid, xs, n, roofline["avg_ms"], args.spmv_reps)
+    log(f"matrix-free stencil spmv: {spmv_stencil['avg_ms']} ms, {spmv_stencil['speedup_vs_csr_spmv']}x the CSR "
+        f"SpMV, bitwise equal: {spmv_stencil['bitwise_equal_to_csr_spmv']}")
     roofline_fp16 = measure_spmv_fp16(ctx, n, nnz, args.spmv_reps)
     if live_traffic and "fp16_traffic_bytes" in live_traffic:
         roofline_fp16["traffic"] = live_traffic["fp16_traffic_bytes"]
@@ -609,6 +634,7 @@ def main():
         "roofline": roofline,
         "roofline_network": roofline_net,
         "roofline_spmv_fp16": roofline_fp16,
+        "spmv_matrix_free": spmv_stencil,
         "cpu_baseline": cpu,
         "setup": setup,
         "solvers": results,

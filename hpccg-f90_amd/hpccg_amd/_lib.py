@@ -88,6 +88,8 @@ _SIGS = {
     "hpccg_time_spmv": (ctypes.c_int, [P, P, P, I32, PD]),
     "hpccg_time_copy": (ctypes.c_int, [P, P, P, I64, I32, PD]),
     "hpccg_time_cnn_apply": (ctypes.c_int, [P, I32, I32, I32, P, P, I32, PD]),
+    "hpccg_spmv_stencil_dev": (ctypes.c_int, [P, I32, I32, I32, P, P]),
+    "hpccg_time_spmv_stencil": (ctypes.c_int, [P, I32, I32, I32, P, P, I32, PD]),
 }
 
 _lib = None
@@ -516,6 +518,16 @@ class Context:
     def time_spmv(self, x: DeviceArray, y: DeviceArray, reps=20):
         ms = ctypes.c_double()
         _check(lib().hpccg_time_spmv(self.h, x.ptr, y.ptr, reps, ctypes.byref(ms)), "time_spmv")
+        return ms.value
+
+    def spmv_stencil_dev(self, dims, x: DeviceArray, y: DeviceArray):
+        """Matrix-free y = A x of generate_matrix's 27-point operator on the grid `dims`."""
+        _check(lib().hpccg_spmv_stencil_dev(self.h, *dims, x.ptr, y.ptr), "spmv_stencil_dev")
+
+    def time_spmv_stencil(self, dims, x: DeviceArray, y: DeviceArray, reps=20):
+        ms = ctypes.c_double()
+        _check(lib().hpccg_time_spmv_stencil(self.h, *dims, x.ptr, y.ptr, reps, ctypes.byref(ms)),
+               "time_spmv_stencil")
         return ms.value
 
     def time_cnn_apply(self, dims, r: DeviceArray, z: DeviceArray, reps=20):

@@ -260,6 +260,15 @@ int hpccg_fcg_learned_dev(hpccg_ctx *ctx, int32_t nx, int32_t ny, int32_t nz, co
  * launches, with its p.Ap epilogue -- and return the mean kernel time in ms (HIP events on the
  * context's stream, the stream the kernel is launched on). */
 int hpccg_time_spmv(hpccg_ctx *ctx, const double *x, double *y, int32_t reps, double *avg_ms);
+/* No reference counterpart (the reference always multiplies by its stored CSR): the
+ * matrix-free form of generate_matrix's operator (SparseMatrix.F90:138-175) on an nx x ny x nz
+ * grid, for comparison with the CSR SpMV.  y = A x from the grid extents alone (27 on the
+ * diagonal, -1 per in-grid neighbour), each row summed in generate_matrix's column order, so y
+ * is bitwise hpccg_spmv_dev's on that matrix; no matrix needs to be resident.  Device arrays. */
+int hpccg_spmv_stencil_dev(hpccg_ctx *ctx, int32_t nx, int32_t ny, int32_t nz, const double *x, double *y);
+/* Mean time in ms of `reps` back-to-back matrix-free SpMVs (HIP events on the context's stream). */
+int hpccg_time_spmv_stencil(hpccg_ctx *ctx, int32_t nx, int32_t ny, int32_t nz, const double *x, double *y,
+                            int32_t reps, double *avg_ms);
 /* A plain device copy of `bytes` bytes, `reps` times: the HBM calibration kernel. */
 int hpccg_time_copy(hpccg_ctx *ctx, void *dst, const void *src, int64_t bytes, int32_t reps,
                     double *avg_ms);
